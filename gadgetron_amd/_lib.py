@@ -1,0 +1,143 @@
+"""ctypes binding of libgtnufft.so (the C ABI declared in include/gt_nufft.h).
+
+The library is built in-tree by gadgetron_amd/csrc/Makefile.  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgtnufft.so")
+
+# Every entry point include/gt_nufft.h declares, in declaration order.
+EXPORTED_SYMBOLS = [
+    "gt_nufft_create",
+    "gt_nufft_destroy",
+    "gt_nufft_preprocess",
+    "gt_nufft_compute",
+    "gt_nufft_mult_MH_M",
+    "gt_nufft_convolve",
+    "gt_nufft_fft",
+    "gt_nufft_deapodize",
+    "gt_nufft_get_matrix_size",
+    "gt_nufft_get_matrix_size_os",
+    "gt_nufft_get_W",
+    "gt_nufft_get_num_samples_frames",
+    "gt_nufft_set_tiling",
+    "gt_nufft_profile",
+    "gt_nufft_profile_read",
+    "gt_last_error",
+    "gt_device_count",
+    "gt_set_device",
+    "gt_malloc",
+    "gt_free",
+    "gt_memcpy_htod",
+    "gt_memcpy_dtoh",
+    "gt_memcpy_dtod",
+    "gt_memset",
+    "gt_stream_synchronize",
+    "gt_device_synchronize",
+]
+
+# gt_status
+GT_OK = 0
+STATUS_NAMES = {
+    0: "GT_OK",
+    1: "GT_ERR_INVALID_ARGUMENT",
+    2: "GT_ERR_NOT_PREPROCESSED",
+    3: "GT_ERR_SHAPE",
+    4: "GT_ERR_RANGE",
+    5: "GT_ERR_UNSUPPORTED",
+    6: "GT_ERR_DEVICE",
+    7: "GT_ERR_FFT",
+    8: "GT_ERR_OUT_OF_MEMORY",
+    9: "GT_ERR_INTERNAL",
+}
+
+
+class GTError(RuntimeError):
+    """Raised for a non-zero gt_status; mirrors the reference's std::runtime_error."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class gt_nufft_desc(ctypes.Structure):
+    _fields_ = [
+        ("D", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("N", ctypes.c_size_t * 4),
+        ("N_os", ctypes.c_size_t * 4),
+        ("os_factor", ctypes.c_double),
+        ("W", ctypes.c_double),
+        ("conv_type", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing; build it with gadgetron_amd/csrc/Makefile "
+            "(or __graft_entry__.build()).  There is no CPU fallback.")
+    L = ctypes.CDLL(LIB_PATH)
+    P, S, I, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+    SP = ctypes.POINTER(ctypes.c_size_t)
+    FP = ctypes.POINTER(ctypes.c_float)
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    sigs = {
+        "gt_nufft_create": [PP, ctypes.POINTER(gt_nufft_desc)],
+        "gt_nufft_destroy": [P],
+        "gt_nufft_preprocess": [P, P, S, S, I, P],
+        "gt_nufft_compute": [P, P, SP, I, P, SP, I, P, S, I, P],
+        "gt_nufft_mult_MH_M": [P, P, P, SP, I, P, S, P],
+        "gt_nufft_convolve": [P, P, SP, I, P, SP, I, I, I, P],
+        "gt_nufft_fft": [P, P, SP, I, I, I, P],
+        "gt_nufft_deapodize": [P, P, SP, I, I, P],
+        "gt_nufft_get_matrix_size": [P, SP],
+        "gt_nufft_get_matrix_size_os": [P, SP],
+        "gt_nufft_get_W": [P, ctypes.POINTER(D)],
+        "gt_nufft_get_num_samples_frames": [P, SP, SP],
+        "gt_nufft_set_tiling": [P, ctypes.POINTER(I), I],
+        "gt_nufft_profile": [P, I],
+        "gt_nufft_profile_read": [P, ctypes.POINTER(D), ctypes.POINTER(I), I],
+        "gt_last_error": [],
+        "gt_device_count": [ctypes.POINTER(I)],
+        "gt_set_device": [I],
+        "gt_malloc": [PP, S],
+        "gt_free": [P],
+        "gt_memcpy_htod": [P, P, S, P],
+        "gt_memcpy_dtoh": [P, P, S, P],
+        "gt_memcpy_dtod": [P, P, S, P],
+        "gt_memset": [P, I, S, P],
+        "gt_stream_synchronize": [P],
+        "gt_device_synchronize": [],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.gt_last_error.restype = ctypes.c_char_p
+    _ = FP
+    _lib = L
+    return _lib
+
+
+def check(status):
+    if status != GT_OK:
+        msg = lib().gt_last_error()
+        raise GTError(status, msg.decode() if msg else "")
+    return status
+
+
+def sizes(seq):
+    seq = [int(v) for v in seq]
+    return (ctypes.c_size_t * max(1, len(seq)))(*seq)
