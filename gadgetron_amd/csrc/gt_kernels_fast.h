@@ -1,0 +1,282 @@
+// gt_kernels_fast.h -- per-trajectory tables, split-tile reduction and row
+// kernels of the 2D hot path (the gridding kernels are in gt_kernels_reg.h).
+//
+// The gridding kernels run one sample at a time per wavefront, with the 64
+// lanes as 32 coils x (real, imaginary), so a sample's control data -- sample
+// index, window position, the 2 x KW Kaiser-Bessel weights -- is uniform
+// across the wave.  preprocess writes it once per trajectory as a table entry
+// per (tile, sample) pair; the kernels read entries with uniform loads, and
+// the window bookkeeping runs on the scalar unit.
+//
+// Grid arrays are addressed through Layout strides, so the same kernels serve
+// the internal row-interleaved layout [y][batch][x] of compute() and the
+// caller's [x, y, batch] layout of convolve().
+#pragma once
+
+#include "gt_kernels.h"
+
+namespace gt {
+
+struct Layout {
+    long long sb, sy, sz;  // element strides of batch, y, z; x stride is 1
+};
+
+constexpr int kLanes = 64;      // one wavefront
+constexpr int kCoilsPerWave = 32;
+constexpr int kSubP = 65;       // LDS floats per cell: 64 lanes + 1 (odd stride, conflict-free rows)
+constexpr int kTabDw = 16;      // dwords per 2D table entry
+constexpr int kTabKW = 7;       // weight slots per dimension in a table entry
+constexpr int kChunk = 4;       // table entries whose loads are issued together
+
+// Table entry (16 dwords):
+//   [0] sample index j (frame * M + m)
+//   [1] packed: bits 0-7 bx (int8), 8-15 by (int8), 16-23 x slot mask, 24-31 y slot mask
+//       bx/by = local coordinate of window slot 0 (tile-local for NC2C, staged-tile-local
+//       for C2NC); bit i of a mask marks slot i as valid (inside the window and, for
+//       NC2C, inside the tile).  Weights of invalid slots are 0.
+//   [2..8] x weights of slots 0..6, [9..15] y weights of slots 0..6
+
+__device__ __forceinline__ uint32_t pack_entry(int bx, int by, uint32_t mx, uint32_t my) {
+    return (uint32_t)(bx & 0xff) | ((uint32_t)(by & 0xff) << 8) | ((mx & 0xff) << 16) | ((my & 0xff) << 24);
+}
+
+// Window of one sample along one dimension with unwrapped cell indices:
+// slots i = 0..KW-1 are cells lo + i, valid (weight != 0) up to floor(p + r).
+template <int KW>
+__device__ __forceinline__ void window_raw(float p, const GridGeom& g, int d, float (&w)[KW], int& lo) {
+    const float r = g.radius;
+    lo = (int)ceilf(p - r);
+    const int hi = (int)floorf(p + r);
+#pragma unroll
+    for (int i = 0; i < KW; i++) {
+        const int c = lo + i;
+        const float u = fabsf((float)c - p);
+        const bool ok = (c <= hi) && !(u > r);
+        w[i] = ok ? kb1(u, g.nos_f[d], g.one_over_W, g.beta[d]) : 0.0f;
+    }
+}
+
+__device__ __forceinline__ int home_tile_1d(float p, const GridGeom& g, int d, float& pw) {
+    pw = (p >= g.nos_f[d]) ? p - g.nos_f[d] : p;  // k = +1/2 sits on the cell of k = -1/2
+    int c = (int)floorf(pw);
+    c = min(max(c, 0), g.nos[d] - 1);
+    return c / g.tile[d];
+}
+
+// Home tile key (one pair per sample) for the C2NC binning.
+__global__ void home_keys_kernel(const float* __restrict__ p, long long nsamp, long long M, GridGeom g,
+                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nsamp) return;
+    float pw;
+    const int tx = home_tile_1d(p[j * 2 + 0], g, 0, pw);
+    const int ty = home_tile_1d(p[j * 2 + 1], g, 1, pw);
+    keys[j] = (uint32_t)(j / M) * (uint32_t)g.ntiles_total + (uint32_t)(tx + g.ntiles[0] * ty);
+    vals[j] = (uint32_t)j;
+}
+
+template <int KW>
+__device__ __forceinline__ void write_entry(uint32_t* t, uint32_t j, uint32_t packed, const float (&wx)[KW],
+                                            const float (&wy)[KW], uint32_t mx, uint32_t my) {
+    t[0] = j;
+    t[1] = packed;
+#pragma unroll
+    for (int i = 0; i < kTabKW; i++) {
+        float a = 0.0f, b = 0.0f;
+        if (i < KW) {
+            a = ((mx >> i) & 1u) ? wx[i < KW ? i : 0] : 0.0f;
+            b = ((my >> i) & 1u) ? wy[i < KW ? i : 0] : 0.0f;
+        }
+        t[2 + i] = __float_as_uint(a);
+        t[2 + kTabKW + i] = __float_as_uint(b);
+    }
+}
+
+// C2NC table: window relative to the staged tile (tile origin - floor(r)).
+template <int KW>
+__global__ void c2nc_table2d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list,
+                                    long long n, GridGeom g, uint32_t* __restrict__ tab) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t j = list[e];
+    const int fr = (int)floorf(g.radius);
+    float wx[KW], wy[KW];
+    int lox, loy;
+    float px, py;
+    const int tx = home_tile_1d(p[(long long)j * 2 + 0], g, 0, px);
+    const int ty = home_tile_1d(p[(long long)j * 2 + 1], g, 1, py);
+    window_raw<KW>(px, g, 0, wx, lox);
+    window_raw<KW>(py, g, 1, wy, loy);
+    uint32_t mx = 0, my = 0;
+#pragma unroll
+    for (int i = 0; i < KW; i++) {
+        mx |= (wx[i] != 0.0f ? 1u : 0u) << i;
+        my |= (wy[i] != 0.0f ? 1u : 0u) << i;
+    }
+    const int bx = lox - (tx * g.tile[0] - fr);
+    const int by = loy - (ty * g.tile[1] - fr);
+    write_entry<KW>(tab + e * kTabDw, j, pack_entry(bx, by, mx, my), wx, wy, mx, my);
+}
+
+// NC2C table: per (tile, sample) pair, the in-tile slots of the window.  For a
+// grid of at least tile + KW cells along each axis the in-tile slots of one
+// axis share a single shift, so local(slot i) = base + i.
+template <int KW>
+__global__ void nc2c_table2d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list,
+                                    const uint32_t* __restrict__ keys, long long n, GridGeom g,
+                                    uint32_t* __restrict__ tab) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t j = list[e];
+    const int tile = (int)(keys[e] % (uint32_t)g.ntiles_total);
+    const int org[2] = {(tile % g.ntiles[0]) * g.tile[0], (tile / g.ntiles[0]) * g.tile[1]};
+    float w[2][KW];
+    int base[2];
+    uint32_t mask[2];
+    for (int d = 0; d < 2; d++) {
+        int lo;
+        window_raw<KW>(p[(long long)j * 2 + d], g, d, w[d], lo);
+        int shift = 0;
+        bool have = false;
+        uint32_t m = 0;
+        for (int i = 0; i < KW; i++) {
+            if (w[d][i] == 0.0f) continue;
+            const int l = wrap_cell(lo + i, g.nos[d]) - org[d];
+            if (l >= 0 && l < g.tile[d]) {
+                if (!have) {
+                    shift = l - i;
+                    have = true;
+                }
+                m |= 1u << i;
+            }
+        }
+        base[d] = shift;
+        mask[d] = m;
+    }
+    write_entry<KW>(tab + e * kTabDw, j, pack_entry(base[0], base[1], mask[0], mask[1]), w[0], w[1], mask[0],
+                    mask[1]);
+}
+
+struct Entry {
+    uint32_t j;
+    int bx, by;
+    uint32_t mx, my;
+};
+
+__device__ __forceinline__ Entry read_entry(const uint32_t* __restrict__ t, float* wx, float* wy) {
+    Entry en;
+    en.j = t[0];
+    const uint32_t pk = t[1];
+    en.bx = (int)(int8_t)(pk & 0xff);
+    en.by = (int)(int8_t)((pk >> 8) & 0xff);
+    en.mx = (pk >> 16) & 0xff;
+    en.my = pk >> 24;
+#pragma unroll
+    for (int i = 0; i < kTabKW; i++) {
+        wx[i] = __uint_as_float(t[2 + i]);
+        wy[i] = __uint_as_float(t[2 + kTabKW + i]);
+    }
+    return en;
+}
+
+// Index into dcw for sample element (coil, j): the reference broadcasts dcw by
+// modulus of its length (cuNDArray_operators.cu:129-145).
+__device__ __forceinline__ long long dcw_index(int coil, uint32_t j, long long FM, long long dcw_len) {
+    if (dcw_len == FM) return j;
+    return ((long long)coil * FM + j) % dcw_len;
+}
+
+__global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restrict__ partials,
+                                                            float* __restrict__ grid,
+                                                            const RedItem* __restrict__ red, int F, int C,
+                                                            GridGeom g, Layout L, int sign, int accumulate) {
+    const RedItem it = red[blockIdx.x];
+    const int T0 = g.tile[0], T1 = g.tile[1];
+    const int TC = T0 * T1;
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    const int t = (int)(it.key % (uint32_t)g.ntiles_total);
+    const int ox = (t % g.ntiles[0]) * T0, oy = (t / g.ntiles[0]) * T1;
+    for (int i = threadIdx.x; i < C * TC * 2; i += blockDim.x) {
+        const int comp = i & 1;
+        const int cell = (i >> 1) % TC;
+        const int cc = (i >> 1) / TC;
+        const int lx = cell % T0, ly = cell / T0;
+        const int gx = ox + lx, gy = oy + ly;
+        if (gx >= g.nos[0] || gy >= g.nos[1]) continue;
+        float v = 0.0f;
+        for (int s = 0; s < it.count; s++)
+            v += partials[(((long long)(it.first + s) * C + cc) * TC + cell) * 2 + comp];
+        if (sign && ((gx + gy) & 1)) v = -v;
+        const long long o = 2 * ((long long)gx + L.sy * gy + L.sb * ((long long)cc * F + f)) + comp;
+        if (accumulate) v += grid[o];
+        grid[o] = v;
+    }
+}
+
+// LDS floats of the C2NC staged tile: (T0 + KW - 1) x (T1 + KW - 1) cells plus
+// KW zeroed pad cells, read (with weight 0) by the last slots of the last row.
+__host__ __device__ inline int c2nc_lds_floats(int T0, int T1, int KW) {
+    return ((T0 + KW - 1) * (T1 + KW - 1) + KW) * kSubP;
+}
+
+// ---------------------------------------------------------------------------
+// Row kernels between the caller's image layout [x, y, batch] and the internal
+// grid layout [y][batch][x]: one workgroup per grid (or image) row.
+// ---------------------------------------------------------------------------
+struct RowGeom {
+    int Nx, Ny;       // image size
+    int Ox, Oy;       // oversampled size
+    int offx, offy;   // pad or crop offsets
+    int nb;           // batches (F * C)
+    const float* fx;  // reciprocal 1-D deapodization filters
+    const float* fy;
+    float scale;
+    int sign;
+};
+
+// grid row r = y * nb + b (internal layout); writes the full row.
+__global__ __launch_bounds__(256) void pad_rows2d_kernel(const float2* __restrict__ img,
+                                                         float2* __restrict__ grid, RowGeom rg) {
+    const int r = blockIdx.x;
+    const int y = r / rg.nb, b = r % rg.nb;
+    const int iy = y - rg.offy;
+    float2* out = grid + (long long)r * rg.Ox;
+    if (iy < 0 || iy >= rg.Ny) {
+        for (int x = threadIdx.x; x < rg.Ox; x += blockDim.x) out[x] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const float2* in = img + ((long long)b * rg.Ny + iy) * rg.Nx;
+    const float sy = rg.scale * rg.fy[y] * ((rg.sign && (y & 1)) ? -1.0f : 1.0f);
+    for (int x = threadIdx.x; x < rg.Ox; x += blockDim.x) {
+        const int ix = x - rg.offx;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (ix >= 0 && ix < rg.Nx) {
+            v = in[ix];
+            const float s = sy * rg.fx[x] * ((rg.sign && (x & 1)) ? -1.0f : 1.0f);
+            v.x *= s;
+            v.y *= s;
+        }
+        out[x] = v;
+    }
+}
+
+// image row r = b * Ny + iy; reads grid row (iy + offy) * nb + b.
+__global__ __launch_bounds__(256) void crop_rows2d_kernel(const float2* __restrict__ grid,
+                                                          float2* __restrict__ img, RowGeom rg) {
+    const int r = blockIdx.x;
+    const int b = r / rg.Ny, iy = r % rg.Ny;
+    const int y = iy + rg.offy;
+    const float2* in = grid + ((long long)y * rg.nb + b) * rg.Ox;
+    float2* out = img + (long long)r * rg.Nx;
+    const float sy = rg.scale * rg.fy[y] * ((rg.sign && (y & 1)) ? -1.0f : 1.0f);
+    for (int ix = threadIdx.x; ix < rg.Nx; ix += blockDim.x) {
+        const int x = ix + rg.offx;
+        float2 v = in[x];
+        const float s = sy * rg.fx[x] * ((rg.sign && (x & 1)) ? -1.0f : 1.0f);
+        v.x *= s;
+        v.y *= s;
+        out[ix] = v;
+    }
+}
+
+}  // namespace gt

@@ -5,11 +5,16 @@
 // cuNFFT_impl specifics (toolboxes/nfft/gpu/cuNFFT.cu) and the gridding
 // convolution object (toolboxes/nfft/gpu/cuGriddingConvolution.cu), re-planned
 // for MI355X:
-//  * preprocess bins samples into grid tiles on the device (count, scan, stable
-//    radix sort), and splits dense tiles into work items, once per trajectory;
-//  * compute runs three launches per direction (fused pad/deapodize/sign,
-//    rocFFT, gridding) with no per-call permutes or timeswitch passes;
-//  * rocFFT plans are created once per batch count and cached;
+//  * preprocess bins the samples into grid tiles on the device (key emission,
+//    scan, stable radix sort), splits dense tiles into work items and, for the
+//    2D hot path, writes the per-sample Kaiser-Bessel tables the gridding
+//    kernels read with uniform loads -- once per trajectory;
+//  * a 2D compute keeps its oversampled grids in the row-interleaved layout
+//    [y][batch][x], so each FFT is two single rocFFT passes (rows, then
+//    columns with batch distance 1) instead of a plan per image, the rows that
+//    are known zero (forward) or cropped away (adjoint) are skipped, and pad /
+//    crop / deapodization / centring / scaling run as one row kernel each;
+//  * rocFFT plans are created once per (pass, batch) and cached;
 //  * workspaces grow on demand and are kept across calls.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -18,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -25,11 +31,14 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/gt_nufft.h"
 #include "gt_kb.h"
 #include "gt_kernels.h"
+#include "gt_kernels_fast.h"
+#include "gt_kernels_reg.h"
 
 namespace {
 
@@ -114,6 +123,41 @@ struct FFTPlan {
         }                                                                      \
     } while (0)
 
+enum FFTKind { FFT_IMAGE = 0, FFT_ROWS = 1, FFT_COLS = 2, FFT_GRID2D = 3 };
+
+// How a 2D compute transforms its row-interleaved grids: two 1-D passes that
+// skip the zero / cropped rows, or one strided 2-D rocFFT plan.  The
+// GT_NUFFT_FFT environment variable ("passes" or "grid2d") overrides the default.
+int fft2d_strategy() {
+    static int mode = [] {
+        const char* e = std::getenv("GT_NUFFT_FFT");
+        if (e && std::string(e) == "passes") return 0;
+        if (e && std::string(e) == "grid2d") return 1;
+        return 1;
+    }();
+    return mode;
+}
+
+// Sample lists of one binning: the tile-sorted sample indices, the
+// matching keys, and the work items over them.
+struct Binning {
+    gt::GridGeom geom{};
+    DevBuf list;      // sample index per entry, sorted by tile key then sample
+    DevBuf table;     // per-entry Kaiser-Bessel table (2D hot path)
+    DevBuf items;     // gt::WorkItem
+    DevBuf red;       // gt::RedItem (split tiles)
+    size_t n = 0;
+    int n_items = 0, n_red = 0, n_partials = 0;
+    void release() {
+        list.release();
+        table.release();
+        items.release();
+        red.release();
+        n = 0;
+        n_items = n_red = n_partials = 0;
+    }
+};
+
 }  // namespace
 
 struct gt_nufft_plan {
@@ -126,29 +170,26 @@ struct gt_nufft_plan {
     int KW = 0;
     int tile_req[3] = {0, 0, 0};
     int max_items_req = 0;
+    bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
 
-    gt::GridGeom geom{};
+    gt::GridGeom geom{};  // geometry for the generic kernels
     std::vector<float> filt_host[3];
     DevBuf filt[3];
 
     // preprocess state
     size_t M = 0, F = 0;
     bool prep_c2nc = false, prep_nc2c = false;
-    DevBuf traj;     // scaled positions [M*F][D]
+    DevBuf traj;  // scaled positions [M*F][D]
     DevBuf flag;
-    DevBuf tile_samples;  // tile-sorted sample indices
-    DevBuf items;         // gt::WorkItem
-    DevBuf red;           // gt::RedItem
-    int n_items = 0, n_red = 0, n_partials = 0;
-    size_t n_pairs = 0;
-    std::vector<uint32_t> tile_counts_host;
+    Binning bn;   // NC2C: (tile, sample) pairs for every tile a window touches
+    Binning bc;   // C2NC: one entry per sample in its home tile (2D hot path)
 
     // workspaces
-    DevBuf grid;      // oversampled grid(s)
+    DevBuf grid;
     DevBuf tmp_samples;
     DevBuf partials;
 
-    std::map<size_t, FFTPlan> ffts;
+    std::map<std::pair<int, size_t>, FFTPlan> ffts;
 
     // stage timing (gt_nufft_profile)
     bool profiling = false;
@@ -184,9 +225,8 @@ struct gt_nufft_plan {
         for (auto& b : filt) b.release();
         traj.release();
         flag.release();
-        tile_samples.release();
-        items.release();
-        red.release();
+        bn.release();
+        bc.release();
         grid.release();
         tmp_samples.release();
         partials.release();
@@ -260,16 +300,7 @@ std::vector<float> deapod_1d(size_t nos, float radius, float one_over_W, float b
 
 void set_device(const gt_nufft_plan* p) { GT_HIP(hipSetDevice(p->device)); }
 
-FFTPlan& get_fft(gt_nufft_plan* p, size_t batch) {
-    auto it = p->ffts.find(batch);
-    if (it != p->ffts.end()) return it->second;
-    std::call_once(g_rocfft_once, [] { rocfft_setup(); });
-    FFTPlan fp;
-    size_t lengths[3] = {p->Nos[0], p->Nos[1], p->Nos[2]};  // fastest dimension first
-    GT_FFT(rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
-                              rocfft_precision_single, p->D, lengths, batch, nullptr));
-    GT_FFT(rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
-                              rocfft_precision_single, p->D, lengths, batch, nullptr));
+void finish_fft_plan(FFTPlan& fp) {
     size_t w1 = 0, w2 = 0;
     GT_FFT(rocfft_plan_get_work_buffer_size(fp.fwd, &w1));
     GT_FFT(rocfft_plan_get_work_buffer_size(fp.bwd, &w2));
@@ -279,15 +310,84 @@ FFTPlan& get_fft(gt_nufft_plan* p, size_t batch) {
         fp.work.reserve(w);
         GT_FFT(rocfft_execution_info_set_work_buffer(fp.info, fp.work.ptr, w));
     }
-    return p->ffts.emplace(batch, std::move(fp)).first->second;
 }
 
-void run_fft(gt_nufft_plan* p, void* data, size_t batch, bool forward, hipStream_t s) {
-    FFTPlan& fp = get_fft(p, batch);
-    StageTimer timer(p, GT_STAGE_FFT, s);
+// FFT_IMAGE: D-dim transform of each image of the caller's layout [x, y, z, batch];
+// FFT_ROWS:  1-D transforms of contiguous rows of length N_os[0];
+// FFT_COLS:  1-D transforms along y of the [y][batch][x] layout (stride batch*N_os[0],
+//            batch distance 1), `batch` = number of images.
+FFTPlan& get_fft(gt_nufft_plan* p, int kind, size_t batch) {
+    const auto key = std::make_pair(kind, batch);
+    auto it = p->ffts.find(key);
+    if (it != p->ffts.end()) return it->second;
+    std::call_once(g_rocfft_once, [] { rocfft_setup(); });
+    FFTPlan fp;
+    if (kind == FFT_IMAGE) {
+        size_t lengths[3] = {p->Nos[0], p->Nos[1], p->Nos[2]};  // fastest dimension first
+        GT_FFT(rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                                  rocfft_precision_single, p->D, lengths, batch, nullptr));
+        GT_FFT(rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                                  rocfft_precision_single, p->D, lengths, batch, nullptr));
+    } else if (kind == FFT_GRID2D) {
+        // 2-D transforms of the [y][batch][x] layout: x stride 1, y stride batch*N_os[0],
+        // one transform per batch at distance N_os[0]
+        rocfft_plan_description desc = nullptr;
+        GT_FFT(rocfft_plan_description_create(&desc));
+        size_t lengths[2] = {p->Nos[0], p->Nos[1]};
+        size_t strides[2] = {1, batch * p->Nos[0]};
+        const size_t dist = p->Nos[0];
+        rocfft_status st = rocfft_plan_description_set_data_layout(
+            desc, rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved, nullptr,
+            nullptr, 2, strides, dist, 2, strides, dist);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                                    rocfft_precision_single, 2, lengths, batch, desc);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                                    rocfft_precision_single, 2, lengths, batch, desc);
+        rocfft_plan_description_destroy(desc);
+        GT_FFT(st);
+    } else {
+        rocfft_plan_description desc = nullptr;
+        GT_FFT(rocfft_plan_description_create(&desc));
+        size_t len, stride, dist, count;
+        if (kind == FFT_ROWS) {
+            len = p->Nos[0];
+            stride = 1;
+            dist = p->Nos[0];
+            count = batch;
+        } else {
+            len = p->Nos[1];
+            stride = batch * p->Nos[0];
+            dist = 1;
+            count = batch * p->Nos[0];
+        }
+        rocfft_status st = rocfft_plan_description_set_data_layout(
+            desc, rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved, nullptr,
+            nullptr, 1, &stride, dist, 1, &stride, dist);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
+                                    rocfft_precision_single, 1, &len, count, desc);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                                    rocfft_precision_single, 1, &len, count, desc);
+        rocfft_plan_description_destroy(desc);
+        GT_FFT(st);
+    }
+    finish_fft_plan(fp);
+    return p->ffts.emplace(key, std::move(fp)).first->second;
+}
+
+void exec_fft(gt_nufft_plan* p, int kind, size_t batch, void* data, bool forward, hipStream_t s) {
+    FFTPlan& fp = get_fft(p, kind, batch);
     GT_FFT(rocfft_execution_info_set_stream(fp.info, s));
     void* buf[1] = {data};
     GT_FFT(rocfft_execute(forward ? fp.fwd : fp.bwd, buf, nullptr, fp.info));
+}
+
+void run_fft(gt_nufft_plan* p, void* data, size_t batch, bool forward, hipStream_t s) {
+    StageTimer timer(p, GT_STAGE_FFT, s);
+    exec_fft(p, FFT_IMAGE, batch, data, forward, s);
 }
 
 unsigned grid_blocks(long long total, int threads = 256) {
@@ -320,6 +420,22 @@ gt::PadCrop padcrop(const gt_nufft_plan* p, bool pad_side, bool in_os, float sca
     return pc;
 }
 
+gt::Layout caller_layout(const gt_nufft_plan* p) {
+    gt::Layout L;
+    L.sy = (long long)p->Nos[0];
+    L.sz = (long long)(p->Nos[0] * p->Nos[1]);
+    L.sb = (long long)p->G();
+    return L;
+}
+
+gt::Layout internal_layout(const gt_nufft_plan* p, size_t nb) {
+    gt::Layout L;
+    L.sb = (long long)p->Nos[0];
+    L.sy = (long long)(nb * p->Nos[0]);
+    L.sz = (long long)(nb * p->Nos[0] * p->Nos[1]);
+    return L;
+}
+
 // ---- kernel dispatch over (D, KW) ----------------------------------------
 template <template <int, int> class L, class... A>
 void dispatch(int D, int KW, A&&... a) {
@@ -333,6 +449,18 @@ void dispatch(int D, int KW, A&&... a) {
 #undef GT_CASES
 #undef GT_CASE
     throw gt_error(GT_ERR_UNSUPPORTED, "unsupported (D, window) combination");
+}
+
+template <template <int> class L, class... A>
+void dispatch_fast(int KW, A&&... a) {
+    switch (KW) {
+    case 3: L<3>::run(std::forward<A>(a)...); return;
+    case 4: L<4>::run(std::forward<A>(a)...); return;
+    case 5: L<5>::run(std::forward<A>(a)...); return;
+    case 6: L<6>::run(std::forward<A>(a)...); return;
+    case 7: L<7>::run(std::forward<A>(a)...); return;
+    default: throw gt_error(GT_ERR_UNSUPPORTED, "unsupported window for the 2D hot path");
+    }
 }
 
 constexpr int kC2NCChunk = 8;
@@ -373,19 +501,151 @@ struct LaunchNC2CAtomic {
     }
 };
 
+template <int KW>
+struct LaunchTablesNC2C {
+    static void run(const float* pos, const uint32_t* list, const uint32_t* keys, long long n,
+                    const gt::GridGeom& g, uint32_t* tab, hipStream_t s) {
+        hipLaunchKernelGGL((gt::nc2c_table2d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           pos, list, keys, n, g, tab);
+    }
+};
+
+template <int KW>
+struct LaunchTablesC2NC {
+    static void run(const float* pos, const uint32_t* list, long long n, const gt::GridGeom& g, uint32_t* tab,
+                    hipStream_t s) {
+        hipLaunchKernelGGL((gt::c2nc_table2d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           pos, list, n, g, tab);
+    }
+};
+
+template <int KW>
+struct LaunchFastNC2C {
+    static void run(const float* samples, float* grid, float* partials, const gt_nufft_plan* p, int C,
+                    const gt::Layout& L, int sign, const float* dcw, long long dcw_len, int accumulate,
+                    hipStream_t s) {
+        const unsigned chunks = (unsigned)((C + gt::kCoilsPerWave - 1) / gt::kCoilsPerWave);
+        hipLaunchKernelGGL((gt::nc2c_reg2d_kernel<KW>), dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s,
+                           samples, grid, partials, p->bn.table.as<uint32_t>(), p->bn.items.as<gt::WorkItem>(),
+                           int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw, dcw_len, accumulate);
+    }
+};
+
+template <int KW>
+struct LaunchFastC2NC {
+    static void run(const float* grid, float* out, const gt_nufft_plan* p, int C, const gt::Layout& L, int sign,
+                    const float* dcw, long long dcw_len, int accumulate, hipStream_t s) {
+        const gt::GridGeom& g = p->bc.geom;
+        const unsigned chunks = (unsigned)((C + gt::kCoilsPerWave - 1) / gt::kCoilsPerWave);
+        const size_t lds = size_t(gt::c2nc_lds_floats(g.tile[0], g.tile[1], KW)) * sizeof(float);
+        hipLaunchKernelGGL((gt::c2nc_multi2d_kernel<KW>), dim3((unsigned)p->bc.n_items, chunks),
+                           dim3(64 * gt::kWaves), lds, s,
+                           grid, out, p->bc.table.as<uint32_t>(), p->bc.items.as<gt::WorkItem>(), int(p->M),
+                           int(p->F), C, g, L, sign, dcw, dcw_len, accumulate);
+    }
+};
+
 int coil_chunk(const gt_nufft_plan* p) {
     // LDS budget per one-wave workgroup: 16 KiB of re/im accumulators
     int cc = int(16384 / (8 * std::max(1, p->geom.tile_cells)));
     return std::max(1, std::min(cc, 32));
 }
 
+gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile) {
+    gt::GridGeom g{};
+    g.D = p->D;
+    g.radius = p->radius;
+    g.one_over_W = p->one_over_W;
+    g.G = (long long)p->G();
+    g.ntiles_total = 1;
+    g.tile_cells = 1;
+    for (int d = 0; d < 3; d++) {
+        if (d < p->D) {
+            g.nos[d] = int(p->Nos[d]);
+            g.nos_f[d] = float(unsigned(p->Nos[d]));
+            g.beta[d] = p->beta[d];
+            // a window of KW cells must touch at most two tiles per dimension
+            const int t = std::min<int>(std::max(tile[d], p->KW), int(p->Nos[d]));
+            g.tile[d] = t;
+            g.ntiles[d] = int((p->Nos[d] + t - 1) / t);
+        } else {
+            g.nos[d] = 1;
+            g.nos_f[d] = 1.0f;
+            g.tile[d] = 1;
+            g.ntiles[d] = 1;
+        }
+        g.ntiles_total *= g.ntiles[d];
+        g.tile_cells *= g.tile[d];
+    }
+    return g;
+}
+
+void setup_geometry(gt_nufft_plan* p) {
+    int tile[3];
+    for (int d = 0; d < 3; d++) {
+        const int def = p->D == 1 ? 256 : (p->D == 2 ? 16 : 8);
+        tile[d] = p->tile_req[d] > 0 ? p->tile_req[d] : def;
+    }
+    p->geom = make_geom(p, tile);
+    // 2D hot path: STANDARD gridding, a window of 3..7 cells, 8 x 8 tiles
+    // (the default; any other requested tile selects the generic kernels) and
+    // a grid of at least tile + window cells per dimension.
+    p->fast2d = false;
+    if (p->D == 2 && p->conv_type == GT_CONV_STANDARD && p->KW >= 3 && p->KW <= gt::kTabKW) {
+        int ft[3];
+        for (int d = 0; d < 3; d++) ft[d] = p->tile_req[d] > 0 ? p->tile_req[d] : gt::kRT;
+        bool ok = true;
+        for (int d = 0; d < 2; d++) {
+            ok &= ft[d] == gt::kRT;
+            ok &= int(p->Nos[d]) >= ft[d] + p->KW;
+        }
+        ok &= size_t(gt::c2nc_lds_floats(gt::kRT, gt::kRT, p->KW)) * sizeof(float) <= 65536;
+        if (ok) {
+            p->fast2d = true;
+            p->bn.geom = make_geom(p, ft);
+            p->bc.geom = make_geom(p, ft);
+        }
+    }
+}
+
 // ---- preprocess ------------------------------------------------------------
-void build_bins(gt_nufft_plan* p, hipStream_t s) {
-    gt::GridGeom& g = p->geom;
+// Sorts (key, sample) pairs by key (stable, so every tile keeps ascending
+// sample order) and returns the per-key counts on the host.
+std::vector<uint32_t> sort_pairs(DevBuf& keys, DevBuf& vals, size_t npairs, uint32_t nkeys, DevBuf& keys_out,
+                                 DevBuf& vals_out, hipStream_t s) {
+    DevBuf temp, hist;
+    keys_out.reserve(std::max<size_t>(npairs, 1) * 4);
+    vals_out.reserve(std::max<size_t>(npairs, 1) * 4);
+    int end_bit = 1;
+    while ((uint64_t(1) << end_bit) < uint64_t(nkeys)) end_bit++;
+    size_t tb = 0;
+    GT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.as<uint32_t>(), keys_out.as<uint32_t>(),
+                                              vals.as<uint32_t>(), vals_out.as<uint32_t>(), (int)npairs, 0,
+                                              end_bit, s));
+    temp.reserve(std::max<size_t>(tb, 1));
+    GT_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, tb, keys.as<uint32_t>(), keys_out.as<uint32_t>(),
+                                              vals.as<uint32_t>(), vals_out.as<uint32_t>(), (int)npairs, 0,
+                                              end_bit, s));
+    hist.reserve(size_t(nkeys) * 4);
+    GT_HIP(hipMemsetAsync(hist.ptr, 0, size_t(nkeys) * 4, s));
+    if (npairs)
+        hipLaunchKernelGGL(gt::histogram_keys_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s,
+                           keys_out.as<uint32_t>(), (long long)npairs, hist.as<uint32_t>());
+    GT_HIP(hipGetLastError());
+    std::vector<uint32_t> counts(nkeys, 0);
+    GT_HIP(hipMemcpyAsync(counts.data(), hist.ptr, size_t(nkeys) * 4, hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    temp.release();
+    hist.release();
+    return counts;
+}
+
+// Window-tile pairs: one (tile, sample) pair for every tile a sample's window touches.
+std::vector<uint32_t> bin_windows(gt_nufft_plan* p, const gt::GridGeom& g, Binning& b, DevBuf* keys_keep,
+                                  hipStream_t s) {
     const long long nsamp = (long long)(p->M * p->F);
     const uint32_t nkeys = uint32_t(p->F) * uint32_t(g.ntiles_total);
-
-    DevBuf counts, offsets, keys, vals, keys_sorted, hist, temp;
+    DevBuf counts, offsets, keys, vals, keys_sorted, temp;
     counts.reserve(nsamp * sizeof(uint32_t));
     offsets.reserve(nsamp * sizeof(uint32_t));
     hipLaunchKernelGGL(gt::count_tiles_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
@@ -402,119 +662,119 @@ void build_bins(gt_nufft_plan* p, hipStream_t s) {
     GT_HIP(hipMemcpyAsync(&last_cnt, counts.as<uint32_t>() + nsamp - 1, 4, hipMemcpyDeviceToHost, s));
     GT_HIP(hipStreamSynchronize(s));
     const size_t npairs = size_t(last_off) + last_cnt;
-    GT_REQUIRE(npairs < (size_t(1) << 32), GT_ERR_UNSUPPORTED, "too many sample-tile pairs");
-    p->n_pairs = npairs;
-
-    keys.reserve(npairs * 4);
-    vals.reserve(npairs * 4);
-    keys_sorted.reserve(npairs * 4);
-    p->tile_samples.reserve(std::max<size_t>(npairs, 1) * 4);
+    GT_REQUIRE(npairs < (size_t(1) << 31), GT_ERR_UNSUPPORTED, "too many sample-tile pairs");
+    keys.reserve(std::max<size_t>(npairs, 1) * 4);
+    vals.reserve(std::max<size_t>(npairs, 1) * 4);
     hipLaunchKernelGGL(gt::emit_pairs_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
                        p->traj.as<float>(), nsamp, (long long)p->M, g, offsets.as<uint32_t>(),
                        keys.as<uint32_t>(), vals.as<uint32_t>());
     GT_HIP(hipGetLastError());
-    int end_bit = 1;
-    while ((uint64_t(1) << end_bit) < uint64_t(nkeys)) end_bit++;
-    tb = 0;
-    GT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(),
-                                              vals.as<uint32_t>(), p->tile_samples.as<uint32_t>(),
-                                              (int)npairs, 0, end_bit, s));
-    temp.reserve(std::max<size_t>(tb, 1));
-    GT_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, tb, keys.as<uint32_t>(), keys_sorted.as<uint32_t>(),
-                                              vals.as<uint32_t>(), p->tile_samples.as<uint32_t>(),
-                                              (int)npairs, 0, end_bit, s));
-    hist.reserve(size_t(nkeys) * 4);
-    GT_HIP(hipMemsetAsync(hist.ptr, 0, size_t(nkeys) * 4, s));
-    if (npairs)
-        hipLaunchKernelGGL(gt::histogram_keys_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s,
-                           keys_sorted.as<uint32_t>(), (long long)npairs, hist.as<uint32_t>());
-    GT_HIP(hipGetLastError());
-    p->tile_counts_host.assign(nkeys, 0);
-    GT_HIP(hipMemcpyAsync(p->tile_counts_host.data(), hist.ptr, size_t(nkeys) * 4, hipMemcpyDeviceToHost, s));
-    GT_HIP(hipStreamSynchronize(s));
+    DevBuf& ko = keys_keep ? *keys_keep : keys_sorted;
+    auto cnt = sort_pairs(keys, vals, npairs, nkeys, ko, b.list, s);
+    b.n = npairs;
+    return cnt;
+}
 
-    // Work items: one per tile, dense tiles split into segments of at most L
-    // samples whose partial tiles are summed in segment order.
-    size_t L = p->max_items_req > 0 ? size_t(p->max_items_req) : 0;
-    if (L == 0) {
-        const double avg = double(npairs) / double(std::max<uint32_t>(nkeys, 1));
-        L = size_t(std::max(512.0, 4.0 * avg));
-    }
+// Home-tile pairs: one pair per sample (2D hot path C2NC).
+std::vector<uint32_t> bin_home(gt_nufft_plan* p, const gt::GridGeom& g, Binning& b, hipStream_t s) {
+    const long long nsamp = (long long)(p->M * p->F);
+    const uint32_t nkeys = uint32_t(p->F) * uint32_t(g.ntiles_total);
+    DevBuf keys, vals, keys_sorted;
+    keys.reserve(nsamp * 4);
+    vals.reserve(nsamp * 4);
+    hipLaunchKernelGGL(gt::home_keys_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
+                       p->traj.as<float>(), nsamp, (long long)p->M, g, keys.as<uint32_t>(), vals.as<uint32_t>());
+    GT_HIP(hipGetLastError());
+    auto cnt = sort_pairs(keys, vals, size_t(nsamp), nkeys, keys_sorted, b.list, s);
+    b.n = size_t(nsamp);
+    return cnt;
+}
+
+// Work items over per-key counts: one per key, keys above L entries split into
+// segments (with partial tiles summed in segment order when allowed).
+void make_items(gt_nufft_plan* p, Binning& b, const std::vector<uint32_t>& counts, size_t L, bool partials,
+                hipStream_t s) {
     std::vector<gt::WorkItem> items;
     std::vector<gt::RedItem> red;
-    items.reserve(nkeys);
+    items.reserve(counts.size());
     uint32_t begin = 0;
     int n_partials = 0;
-    for (uint32_t k = 0; k < nkeys; k++) {
-        const uint32_t n = p->tile_counts_host[k];
+    for (uint32_t k = 0; k < uint32_t(counts.size()); k++) {
+        const uint32_t n = counts[k];
+        if (n == 0 && !partials) continue;  // C2NC: nothing to output for an empty tile
         if (n <= L) {
             items.push_back({k, begin, begin + n, -1});
+        } else if (!partials) {
+            for (uint32_t b0 = 0; b0 < n; b0 += uint32_t(L))
+                items.push_back({k, begin + b0, begin + std::min<uint32_t>(n, b0 + uint32_t(L)), -1});
         } else {
             const uint32_t nseg = uint32_t((n + L - 1) / L);
             red.push_back({k, n_partials, int32_t(nseg), 0});
             for (uint32_t sgm = 0; sgm < nseg; sgm++) {
-                const uint32_t b = begin + uint32_t(sgm * L);
-                const uint32_t e = std::min<uint32_t>(begin + n, uint32_t(b + L));
-                items.push_back({k, b, e, n_partials++});
+                const uint32_t b0 = begin + uint32_t(sgm * L);
+                const uint32_t e0 = std::min<uint32_t>(begin + n, uint32_t(b0 + L));
+                items.push_back({k, b0, e0, n_partials++});
             }
         }
         begin += n;
     }
     // Longest items first: the dispatcher hands out workgroups in index order.
-    std::stable_sort(items.begin(), items.end(), [](const gt::WorkItem& a, const gt::WorkItem& b) {
-        return (a.end - a.begin) > (b.end - b.begin);
+    std::stable_sort(items.begin(), items.end(), [](const gt::WorkItem& a, const gt::WorkItem& b2) {
+        return (a.end - a.begin) > (b2.end - b2.begin);
     });
-    p->n_items = int(items.size());
-    p->n_red = int(red.size());
-    p->n_partials = n_partials;
-    p->items.reserve(std::max<size_t>(items.size(), 1) * sizeof(gt::WorkItem));
-    p->red.reserve(std::max<size_t>(red.size(), 1) * sizeof(gt::RedItem));
-    GT_HIP(hipMemcpyAsync(p->items.ptr, items.data(), items.size() * sizeof(gt::WorkItem),
-                          hipMemcpyHostToDevice, s));
+    if (items.empty()) items.push_back({0, 0, 0, -2});  // keep one launchable (no-op) item
+    b.n_items = int(items.size());
+    b.n_red = int(red.size());
+    b.n_partials = n_partials;
+    b.items.reserve(items.size() * sizeof(gt::WorkItem));
+    b.red.reserve(std::max<size_t>(red.size(), 1) * sizeof(gt::RedItem));
+    GT_HIP(hipMemcpyAsync(b.items.ptr, items.data(), items.size() * sizeof(gt::WorkItem), hipMemcpyHostToDevice,
+                          s));
     if (!red.empty())
-        GT_HIP(hipMemcpyAsync(p->red.ptr, red.data(), red.size() * sizeof(gt::RedItem),
-                              hipMemcpyHostToDevice, s));
+        GT_HIP(hipMemcpyAsync(b.red.ptr, red.data(), red.size() * sizeof(gt::RedItem), hipMemcpyHostToDevice, s));
     GT_HIP(hipStreamSynchronize(s));
-    counts.release();
-    offsets.release();
-    keys.release();
-    vals.release();
-    keys_sorted.release();
-    hist.release();
-    temp.release();
 }
 
-void setup_geometry(gt_nufft_plan* p) {
-    gt::GridGeom& g = p->geom;
-    g = gt::GridGeom{};
-    g.D = p->D;
-    g.radius = p->radius;
-    g.one_over_W = p->one_over_W;
-    g.G = (long long)p->G();
-    const int def2[3] = {16, 16, 1};
-    const int def3[3] = {8, 8, 8};
-    g.ntiles_total = 1;
-    g.tile_cells = 1;
-    for (int d = 0; d < 3; d++) {
-        if (d < p->D) {
-            g.nos[d] = int(p->Nos[d]);
-            g.nos_f[d] = float(unsigned(p->Nos[d]));
-            g.beta[d] = p->beta[d];
-            int t = p->tile_req[d] > 0 ? p->tile_req[d]
-                                       : (p->D == 1 ? 256 : (p->D == 2 ? def2[d] : def3[d]));
-            // a window of KW cells must touch at most two tiles per dimension
-            t = std::min<int>(std::max(t, p->KW), int(p->Nos[d]));
-            g.tile[d] = t;
-            g.ntiles[d] = int((p->Nos[d] + t - 1) / t);
-        } else {
-            g.nos[d] = 1;
-            g.nos_f[d] = 1.0f;
-            g.tile[d] = 1;
-            g.ntiles[d] = 1;
-        }
-        g.ntiles_total *= g.ntiles[d];
-        g.tile_cells *= g.tile[d];
+// Entries per work item.  A wavefront walks its entries serially, so the
+// longest item bounds the kernel time; splitting an NC2C tile costs one
+// partial tile write and read, splitting a C2NC tile one more halo staging.
+size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& counts, size_t npairs,
+                      bool nc2c) {
+    if (p->max_items_req > 0) return size_t(p->max_items_req);
+    if (p->fast2d) return nc2c ? 256 : 128;
+    const double avg = double(npairs) / double(std::max<size_t>(counts.size(), 1));
+    return size_t(std::max(256.0, 4.0 * avg));
+}
+
+void build_nc2c(gt_nufft_plan* p, hipStream_t s) {
+    Binning& b = p->bn;
+    if (p->fast2d) {
+        DevBuf keys_sorted;
+        auto counts = bin_windows(p, b.geom, b, &keys_sorted, s);
+        b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
+        if (b.n)
+            dispatch_fast<LaunchTablesNC2C>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(),
+                                            keys_sorted.as<uint32_t>(), (long long)b.n, b.geom,
+                                            b.table.as<uint32_t>(), s);
+        GT_HIP(hipGetLastError());
+        make_items(p, b, counts, segment_length(p, counts, b.n, true), true, s);
+        keys_sorted.release();
+    } else {
+        b.geom = p->geom;
+        auto counts = bin_windows(p, b.geom, b, nullptr, s);
+        make_items(p, b, counts, segment_length(p, counts, b.n, true), true, s);
     }
+}
+
+void build_c2nc(gt_nufft_plan* p, hipStream_t s) {
+    if (!p->fast2d) return;  // the generic C2NC gathers per sample without binning
+    Binning& b = p->bc;
+    auto counts = bin_home(p, b.geom, b, s);
+    b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
+    dispatch_fast<LaunchTablesC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n, b.geom,
+                                    b.table.as<uint32_t>(), s);
+    GT_HIP(hipGetLastError());
+    make_items(p, b, counts, segment_length(p, counts, b.n, false), false, s);
 }
 
 // ---- compute building blocks ----------------------------------------------
@@ -558,46 +818,62 @@ void check_batches(const gt_nufft_plan* p, Shape& sh, size_t sample_elems) {
     sh.C = sh.nb / p->F;
 }
 
-void conv_c2nc(gt_nufft_plan* p, const void* grid, void* out, size_t C, int sign, const float* dcw,
-               size_t dcw_len, bool accumulate, hipStream_t s) {
+void conv_c2nc(gt_nufft_plan* p, const void* grid, const gt::Layout& L, void* out, size_t C, int sign,
+               const float* dcw, size_t dcw_len, bool accumulate, hipStream_t s) {
     GT_REQUIRE(p->prep_c2nc, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for C2NC");
     StageTimer timer(p, GT_STAGE_C2NC, s);
-    dispatch<LaunchC2NC>(p->D, p->KW, (const float2*)grid, (float2*)out, p->traj.as<float>(), int(p->M),
-                         int(p->F), int(C), p->geom, sign, dcw, (long long)std::max<size_t>(dcw_len, 1),
-                         accumulate ? 1 : 0, s);
+    const long long dl = (long long)std::max<size_t>(dcw_len, 1);
+    if (p->fast2d) {
+        dispatch_fast<LaunchFastC2NC>(p->KW, (const float*)grid, (float*)out, p, int(C), L, sign, dcw, dl,
+                                      accumulate ? 1 : 0, s);
+    } else {
+        dispatch<LaunchC2NC>(p->D, p->KW, (const float2*)grid, (float2*)out, p->traj.as<float>(), int(p->M),
+                             int(p->F), int(C), p->geom, sign, dcw, dl, accumulate ? 1 : 0, s);
+    }
     GT_HIP(hipGetLastError());
 }
 
-void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, size_t C, int sign, const float* dcw,
-               size_t dcw_len, bool accumulate, hipStream_t s) {
+void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layout& L, size_t C, int sign,
+               const float* dcw, size_t dcw_len, bool accumulate, hipStream_t s) {
     GT_REQUIRE(p->prep_nc2c, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for NC2C");
-    if (p->conv_type != GT_CONV_ATOMIC && p->n_partials > 0)
-        p->partials.reserve(size_t(p->n_partials) * C * p->geom.tile_cells * sizeof(float2));
+    Binning& b = p->bn;
+    if (p->conv_type != GT_CONV_ATOMIC && b.n_partials > 0)
+        p->partials.reserve(size_t(b.n_partials) * C * b.geom.tile_cells * sizeof(float2));
     StageTimer timer(p, GT_STAGE_NC2C, s);
     const long long dl = (long long)std::max<size_t>(dcw_len, 1);
     if (p->conv_type == GT_CONV_ATOMIC) {
-        if (!accumulate)
-            GT_HIP(hipMemsetAsync(grid, 0, p->G() * p->F * C * sizeof(float2), s));
+        if (!accumulate) GT_HIP(hipMemsetAsync(grid, 0, p->G() * p->F * C * sizeof(float2), s));
         dispatch<LaunchNC2CAtomic>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->traj.as<float>(),
                                    int(p->M), int(p->F), int(C), p->geom, sign, dcw, dl, s);
         GT_HIP(hipGetLastError());
         return;
     }
+    if (p->fast2d) {
+        dispatch_fast<LaunchFastNC2C>(p->KW, (const float*)samples, (float*)grid, p->partials.as<float>(), p,
+                                      int(C), L, sign, dcw, dl, accumulate ? 1 : 0, s);
+        GT_HIP(hipGetLastError());
+        if (b.n_red > 0) {
+            hipLaunchKernelGGL(gt::nc2c_reduce2d_kernel, dim3((unsigned)b.n_red), dim3(256), 0, s,
+                               p->partials.as<float>(), (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                               b.geom, L, sign, accumulate ? 1 : 0);
+            GT_HIP(hipGetLastError());
+        }
+        return;
+    }
     const int CC = coil_chunk(p);
     dispatch<LaunchNC2CTile>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->partials.as<float2>(),
-                             p->traj.as<float>(), p->tile_samples.as<uint32_t>(),
-                             p->items.as<gt::WorkItem>(), p->n_items, int(p->M), int(p->F), int(C), CC,
-                             p->geom, sign, dcw, dl, accumulate ? 1 : 0, s);
+                             p->traj.as<float>(), b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), b.n_items,
+                             int(p->M), int(p->F), int(C), CC, b.geom, sign, dcw, dl, accumulate ? 1 : 0, s);
     GT_HIP(hipGetLastError());
-    if (p->n_red > 0) {
-        hipLaunchKernelGGL(gt::nc2c_reduce_kernel, dim3((unsigned)p->n_red), dim3(256), 0, s,
-                           p->partials.as<float2>(), (float2*)grid, p->red.as<gt::RedItem>(), int(p->F),
-                           int(C), p->geom, sign, accumulate ? 1 : 0);
+    if (b.n_red > 0) {
+        hipLaunchKernelGGL(gt::nc2c_reduce_kernel, dim3((unsigned)b.n_red), dim3(256), 0, s,
+                           p->partials.as<float2>(), (float2*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                           b.geom, sign, accumulate ? 1 : 0);
         GT_HIP(hipGetLastError());
     }
 }
 
-// Whole-grid pad (or copy) + deapodize + centring sign + scale.
+// Whole-grid pad (or copy) + deapodize + centring sign + scale, caller layout.
 void pad_deapod(gt_nufft_plan* p, const void* img, void* grid, size_t nb, bool in_os, float scale,
                 hipStream_t s) {
     gt::PadCrop pc = padcrop(p, true, in_os, scale, 1);
@@ -618,6 +894,73 @@ void crop_deapod(gt_nufft_plan* p, const void* grid, void* img, size_t nb, bool 
     GT_HIP(hipGetLastError());
 }
 
+gt::RowGeom row_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float scale) {
+    gt::RowGeom rg{};
+    rg.Nx = int(p->N[0]);
+    rg.Ny = int(p->N[1]);
+    rg.Ox = int(p->Nos[0]);
+    rg.Oy = int(p->Nos[1]);
+    rg.offx = pad_side ? int(p->Nos[0] / 2 - p->N[0] / 2) : int((p->Nos[0] - p->N[0]) >> 1);
+    rg.offy = pad_side ? int(p->Nos[1] / 2 - p->N[1] / 2) : int((p->Nos[1] - p->N[1]) >> 1);
+    rg.nb = int(nb);
+    rg.fx = p->filt[0].as<float>();
+    rg.fy = p->filt[1].as<float>();
+    rg.scale = scale;
+    rg.sign = 1;
+    return rg;
+}
+
+// 2D hot path, image side of size N: internal layout [y][batch][x].
+void compute_fast2d(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const float* dcw, size_t dcw_len,
+                    int mode, float scale, hipStream_t s) {
+    const size_t nb = sh.nb;
+    const size_t Ox = p->Nos[0];
+    p->grid.reserve(nb * p->G() * sizeof(float2));
+    float2* g = p->grid.as<float2>();
+    const gt::Layout L = internal_layout(p, nb);
+    if (mode == GT_FORWARDS_C2NC || mode == GT_BACKWARDS_C2NC) {
+        const bool fwd = mode == GT_FORWARDS_C2NC;
+        gt::RowGeom rg = row_geom(p, nb, true, scale);
+        {
+            StageTimer timer(p, GT_STAGE_PAD, s);
+            hipLaunchKernelGGL(gt::pad_rows2d_kernel, dim3((unsigned)(p->Nos[1] * nb)), dim3(256), 0, s,
+                               (const float2*)in, g, rg);
+            GT_HIP(hipGetLastError());
+        }
+        {
+            StageTimer timer(p, GT_STAGE_FFT, s);
+            if (fft2d_strategy() == 1) {
+                exec_fft(p, FFT_GRID2D, nb, g, fwd, s);
+            } else {
+                // rows outside the padded image are zero: transform only the image rows
+                exec_fft(p, FFT_ROWS, p->N[1] * nb, g + size_t(rg.offy) * nb * Ox, fwd, s);
+                exec_fft(p, FFT_COLS, nb, g, fwd, s);
+            }
+        }
+        conv_c2nc(p, g, L, out, sh.C, 1, dcw, dcw_len, false, s);
+    } else {
+        const bool fwd = mode == GT_FORWARDS_NC2C;
+        gt::RowGeom rg = row_geom(p, nb, false, scale);
+        conv_nc2c(p, in, g, L, sh.C, 1, dcw, dcw_len, false, s);
+        {
+            StageTimer timer(p, GT_STAGE_FFT, s);
+            if (fft2d_strategy() == 1) {
+                exec_fft(p, FFT_GRID2D, nb, g, fwd, s);
+            } else {
+                exec_fft(p, FFT_COLS, nb, g, fwd, s);
+                // only the rows kept by the crop are needed
+                exec_fft(p, FFT_ROWS, p->N[1] * nb, g + size_t(rg.offy) * nb * Ox, fwd, s);
+            }
+        }
+        {
+            StageTimer timer(p, GT_STAGE_CROP, s);
+            hipLaunchKernelGGL(gt::crop_rows2d_kernel, dim3((unsigned)(p->N[1] * nb)), dim3(256), 0, s,
+                               (const float2*)g, (float2*)out, rg);
+            GT_HIP(hipGetLastError());
+        }
+    }
+}
+
 void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int in_ndim, void* out,
                   const size_t* out_dims, int out_ndim, const float* dcw, size_t dcw_len, int mode,
                   hipStream_t s) {
@@ -628,12 +971,15 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
     Shape sh = image_in ? image_shape(p, in_dims, in_ndim, true) : image_shape(p, out_dims, out_ndim, true);
     const size_t selems = image_in ? elements(out_dims, out_ndim) : elements(in_dims, in_ndim);
     check_batches(p, sh, selems);
-    GT_REQUIRE(p->prep_c2nc || !(image_in || mode == GT_BACKWARDS_C2NC), GT_ERR_NOT_PREPROCESSED,
-               "plan not preprocessed for C2NC");
     set_device(p);
     const size_t G = p->G();
     const float gs = centring_global_sign(p);
     const float scale = gs / std::sqrt(float(G));  // do_scale=true, NFFT.hpp:386-418
+    if (p->fast2d && !sh.oversampled) {
+        compute_fast2d(p, in, out, sh, dcw, dcw_len, mode, scale, s);
+        return;
+    }
+    const gt::Layout L = caller_layout(p);
     switch (mode) {
     case GT_FORWARDS_C2NC:
     case GT_BACKWARDS_C2NC: {
@@ -642,7 +988,7 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
         p->grid.reserve(sh.nb * G * sizeof(float2));
         pad_deapod(p, in, p->grid.ptr, sh.nb, sh.oversampled, scale, s);
         run_fft(p, p->grid.ptr, sh.nb, mode == GT_FORWARDS_C2NC, s);
-        conv_c2nc(p, p->grid.ptr, out, sh.C, 1, dcw, dcw_len, false, s);
+        conv_c2nc(p, p->grid.ptr, L, out, sh.C, 1, dcw, dcw_len, false, s);
         break;
     }
     case GT_BACKWARDS_NC2C:
@@ -654,15 +1000,13 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
             p->grid.reserve(sh.nb * G * sizeof(float2));
             g = p->grid.ptr;
         }
-        conv_nc2c(p, in, g, sh.C, 1, dcw, dcw_len, false, s);
+        conv_nc2c(p, in, g, L, sh.C, 1, dcw, dcw_len, false, s);
         run_fft(p, g, sh.nb, mode == GT_FORWARDS_NC2C, s);
         crop_deapod(p, g, out, sh.nb, sh.oversampled, scale, s);
         break;
     }
     }
 }
-
-std::vector<size_t> dims_vec(const size_t* d, int n) { return std::vector<size_t>(d, d + n); }
 
 }  // namespace
 
@@ -754,6 +1098,8 @@ gt_status gt_nufft_set_tiling(gt_nufft_plan* p, const int* tile, int max_samples
         setup_geometry(p);
         int cells = p->geom.tile_cells;
         GT_REQUIRE(size_t(2) * 4 * cells <= 65536, GT_ERR_INVALID_ARGUMENT, "tile does not fit in LDS");
+        p->prep_c2nc = p->prep_nc2c = false;  // bins depend on the tiles: preprocess again
+        p->M = p->F = 0;
     });
 }
 
@@ -783,9 +1129,12 @@ gt_status gt_nufft_preprocess(gt_nufft_plan* p, const void* traj, size_t M, size
         GT_REQUIRE(!bad, GT_ERR_RANGE, "trajectory out of range [-1/2, 1/2] (or not finite)");
         p->M = M;
         p->F = F;
-        if (mode == GT_PREP_C2NC || mode == GT_PREP_ALL) p->prep_c2nc = true;
+        if (mode == GT_PREP_C2NC || mode == GT_PREP_ALL) {
+            build_c2nc(p, s);
+            p->prep_c2nc = true;
+        }
         if (mode == GT_PREP_NC2C || mode == GT_PREP_ALL) {
-            if (p->conv_type == GT_CONV_STANDARD) build_bins(p, s);
+            if (p->conv_type == GT_CONV_STANDARD) build_nc2c(p, s);
             p->prep_nc2c = true;
         }
     });
@@ -825,8 +1174,9 @@ gt_status gt_nufft_convolve(gt_nufft_plan* p, const void* in, const size_t* in_d
         Shape sh = c2nc ? image_shape(p, in_dims, in_ndim, false) : image_shape(p, out_dims, out_ndim, false);
         check_batches(p, sh, c2nc ? elements(out_dims, out_ndim) : elements(in_dims, in_ndim));
         set_device(p);
-        if (c2nc) conv_c2nc(p, in, out, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
-        else conv_nc2c(p, in, out, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
+        const gt::Layout L = caller_layout(p);
+        if (c2nc) conv_c2nc(p, in, L, out, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
+        else conv_nc2c(p, in, out, L, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
     });
 }
 
