@@ -1,0 +1,311 @@
+// gt_fft.h -- the 2D centred FFT of the hot path, fused with pad / crop and
+// deapodization (cuNFFT_impl::fft + deapodize, cuNFFT.cu:57-92, and the pad /
+// crop passes of NFFT_plan::compute, NFFT.hpp:125-184).
+//
+// The grid is the row-interleaved layout [y][batch][x] of compute().  Of the
+// oversampled grid only the band of rows holding the image (forward) or kept
+// by the crop (backward) carries information, so the 2D FFT runs as
+//   forward : image rows --pad, deapodize, sign, scale, row FFT--> band rows
+//             band rows (others are zero) --column FFT--> all rows
+//   backward: all rows --column FFT--> band rows
+//             band rows --row FFT, crop, deapodize, sign, scale--> image rows
+// and the rows outside the band are neither written nor read.  The centring
+// sign (-1)^(x+y) is applied by the neighbouring kernels (pad / crop here, the
+// gridding kernels on the grid side), so the transforms are plain DFTs,
+// e^{-2 pi i k n / L} forwards and e^{+2 pi i k n / L} backwards, unscaled.
+//
+// Each 1D transform is a Stockham autosort FFT in LDS (radices 8, 4, 2, 3, 5,
+// chosen at compile time), in place through registers: a pass reads its
+// butterfly inputs, synchronizes, and writes its outputs.  Twiddles come from
+// a per-length table exp(-2 pi i m / L), m < L, built on the host in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gt_kernels_fast.h"
+
+namespace gt {
+
+constexpr int fft_next_radix(int m) {
+    return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : 0;
+}
+
+// True if L factors into the radices above (and is one of the compiled lengths).
+constexpr bool fft_factorable(int L) {
+    int m = L;
+    while (m > 1) {
+        const int r = fft_next_radix(m);
+        if (r == 0) return false;
+        m /= r;
+    }
+    return true;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (s = -1, forwards) or +i (s = +1, backwards)
+__device__ __forceinline__ float2 cmul_i(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// In-register DFT of R points; s = -1 forwards, +1 backwards.
+template <int R>
+__device__ __forceinline__ void dft(float2 (&x)[R], float s);
+
+template <>
+__device__ __forceinline__ void dft<2>(float2 (&x)[2], float) {
+    const float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void dft<3>(float2 (&x)[3], float s) {
+    // w = e^{s 2 pi i / 3} = -1/2 + s i sqrt(3)/2
+    const float c = -0.5f, d = s * 0.86602540378443864676f;
+    const float2 t = cadd(x[1], x[2]);
+    const float2 u = csub(x[1], x[2]);
+    const float2 m = make_float2(x[0].x + c * t.x, x[0].y + c * t.y);
+    const float2 n = make_float2(-d * u.y, d * u.x);  // i d u
+    x[0] = cadd(x[0], t);
+    x[1] = cadd(m, n);
+    x[2] = csub(m, n);
+}
+
+template <>
+__device__ __forceinline__ void dft<4>(float2 (&x)[4], float s) {
+    const float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+    const float2 c = cadd(x[1], x[3]), d = cmul_i(csub(x[1], x[3]), -s);
+    // X1 = b + w d' with w^1 = e^{s i pi/2} = s i:  (x1 - x3) * s i
+    x[0] = cadd(a, c);
+    x[2] = csub(a, c);
+    x[1] = csub(b, d);
+    x[3] = cadd(b, d);
+}
+
+template <>
+__device__ __forceinline__ void dft<5>(float2 (&x)[5], float s) {
+    // cos/sin of 2 pi / 5 and 4 pi / 5
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = s * 0.95105651629515357212f, s2 = s * 0.58778525229247312917f;
+    const float2 t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
+    const float2 u1 = csub(x[1], x[4]), u2 = csub(x[2], x[3]);
+    const float2 a1 = make_float2(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
+    const float2 a2 = make_float2(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
+    // i (s1 u1 + s2 u2) and i (s2 u1 - s1 u2)
+    const float2 b1 = make_float2(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
+    const float2 b2 = make_float2(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
+    x[0] = cadd(x[0], cadd(t1, t2));
+    x[1] = cadd(a1, b1);
+    x[4] = csub(a1, b1);
+    x[2] = cadd(a2, b2);
+    x[3] = csub(a2, b2);
+}
+
+template <>
+__device__ __forceinline__ void dft<8>(float2 (&x)[8], float s) {
+    // radix-2 over even/odd, then radix-4 halves
+    float2 e[4] = {x[0], x[2], x[4], x[6]};
+    float2 o[4] = {x[1], x[3], x[5], x[7]};
+    dft<4>(e, s);
+    dft<4>(o, s);
+    const float h = 0.70710678118654752440f;
+    // o[k] *= e^{s 2 pi i k / 8}
+    o[1] = make_float2(h * (o[1].x - s * o[1].y), h * (o[1].y + s * o[1].x));
+    o[2] = cmul_i(o[2], s);
+    o[3] = make_float2(h * (-o[3].x - s * o[3].y), h * (-o[3].y + s * o[3].x));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        x[k] = cadd(e[k], o[k]);
+        x[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+// LDS slot of element i of a sequence: contiguous sequences (row transforms)
+// get one pad slot per 8 elements, so the stride-R writes of a pass spread
+// over the banks; interleaved sequences (columns) use a plain stride.
+template <bool PADR>
+__device__ __forceinline__ int fft_slot(int i, int estride) {
+    return PADR ? i + (i >> 3) : i * estride;
+}
+template <int L, bool PADR>
+constexpr int fft_slots() {
+    return PADR ? L + L / 8 : L;
+}
+
+// One Stockham pass of radix R on sequences of length L held in LDS (element
+// i of this thread's sequence at buf[fft_slot(i)]).  NS = product of the
+// radices already applied; t = 0..TPS-1 is the thread's index within its
+// sequence.  Every thread of the block takes part (two barriers).
+template <int L, int R, int NS, int TPS, bool PADR>
+__device__ __forceinline__ void fft_pass(float2* buf, int estride, int t, const float2* __restrict__ tw, float s) {
+    constexpr int NB = L / R;
+    constexpr int PER = (NB + TPS - 1) / TPS;
+    float2 x[PER][R];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int j = t + i * TPS;
+        if (j < NB) {
+#pragma unroll
+            for (int q = 0; q < R; q++) x[i][q] = buf[fft_slot<PADR>(j + q * NB, estride)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int j = t + i * TPS;
+        if (j < NB) {
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int q = 1; q < R; q++) {
+                    float2 w = tw[k * q * (L / (NS * R))];  // k q < NS R: index below L
+                    w.y *= -s;  // the table holds e^{-i theta}; backwards conjugates
+                    x[i][q] = cmul(x[i][q], w);
+                }
+            }
+            dft<R>(x[i], s);
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; q++) buf[fft_slot<PADR>(base + q * NS, estride)] = x[i][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int NS, int TPS, bool PADR>
+__device__ __forceinline__ void fft_passes(float2* buf, int estride, int t, const float2* __restrict__ tw, float s) {
+    if constexpr (NS < L) {
+        constexpr int R = fft_next_radix(L / NS);
+        static_assert(R > 0, "FFT length must factor into 2, 3, 5");
+        fft_pass<L, R, NS, TPS, PADR>(buf, estride, t, tw, s);
+        fft_passes<L, NS * R, TPS, PADR>(buf, estride, t, tw, s);
+    }
+}
+
+// Twiddle table of length L into LDS (all threads of the block).
+template <int L>
+__device__ __forceinline__ void load_twiddles(float2* sw, const float2* __restrict__ tw) {
+    for (int i = threadIdx.x; i < L; i += blockDim.x) sw[i] = tw[i];
+}
+
+struct FftGeom {
+    int Nx, Ny;         // image size
+    int offx, offy;     // band offset (pad offset forwards, crop offset backwards)
+    int nb;             // batches F * C
+    const float* fx;    // reciprocal deapodization filters (length Ox, Oy)
+    const float* fy;
+    float scale;        // 1/sqrt(G) times the centring global sign
+};
+
+constexpr int kFftRowThreads = 64;  // threads per row transform
+constexpr int kFftRows = 4;         // rows per workgroup
+constexpr int kFftColW = 16;        // columns per workgroup (one 128-byte run of a row)
+
+// Forwards: image row (b, iy) -> grid row (iy + offy) * nb + b, padded to Ox
+// at offx, deapodized, centring-signed and scaled, then transformed along x.
+template <int OX>
+__global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_pad_kernel(
+    const float2* __restrict__ img, float2* __restrict__ grid, FftGeom fg, const float2* __restrict__ tw,
+    float s) {
+    constexpr int NS = fft_slots<OX, true>();
+    __shared__ float2 sbuf[kFftRows][NS];
+    __shared__ float2 stw[OX];
+    load_twiddles<OX>(stw, tw);
+    const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
+    const int r = blockIdx.x * kFftRows + rl;  // image row b * Ny + iy
+    const int nrows = fg.nb * fg.Ny;
+    const bool live = r < nrows;
+    const int b = live ? r / fg.Ny : 0, iy = live ? r % fg.Ny : 0;
+    const int y = iy + fg.offy;
+    const float sy = fg.scale * fg.fy[y] * ((y & 1) ? -1.0f : 1.0f);
+    const float2* in = img + ((long long)b * fg.Ny + iy) * fg.Nx;
+    float2* sb = &sbuf[rl][0];
+    for (int x = t; x < OX; x += kFftRowThreads) {
+        const int ix = x - fg.offx;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (live && ix >= 0 && ix < fg.Nx) {
+            v = in[ix];
+            const float f = sy * fg.fx[x] * ((x & 1) ? -1.0f : 1.0f);
+            v.x *= f;
+            v.y *= f;
+        }
+        sb[fft_slot<true>(x, 1)] = v;
+    }
+    __syncthreads();
+    fft_passes<OX, 1, kFftRowThreads, true>(sb, 1, t, stw, s);
+    if (!live) return;
+    float2* out = grid + ((long long)y * fg.nb + b) * OX;
+    for (int x = t; x < OX; x += kFftRowThreads) out[x] = sb[fft_slot<true>(x, 1)];
+}
+
+// Backwards: grid row (iy + offy) * nb + b, transformed along x, cropped at
+// offx, deapodized, centring-signed and scaled -> image row (b, iy).
+template <int OX>
+__global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_crop_kernel(
+    const float2* __restrict__ grid, float2* __restrict__ img, FftGeom fg, const float2* __restrict__ tw,
+    float s) {
+    constexpr int NS = fft_slots<OX, true>();
+    __shared__ float2 sbuf[kFftRows][NS];
+    __shared__ float2 stw[OX];
+    load_twiddles<OX>(stw, tw);
+    const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
+    const int r = blockIdx.x * kFftRows + rl;
+    const int nrows = fg.nb * fg.Ny;
+    const bool live = r < nrows;
+    const int b = live ? r / fg.Ny : 0, iy = live ? r % fg.Ny : 0;
+    const int y = iy + fg.offy;
+    const float2* in = grid + ((long long)y * fg.nb + b) * OX;
+    float2* sb = &sbuf[rl][0];
+    for (int x = t; x < OX; x += kFftRowThreads)
+        sb[fft_slot<true>(x, 1)] = live ? in[x] : make_float2(0.0f, 0.0f);
+    __syncthreads();
+    fft_passes<OX, 1, kFftRowThreads, true>(sb, 1, t, stw, s);
+    if (!live) return;
+    const float sy = fg.scale * fg.fy[y] * ((y & 1) ? -1.0f : 1.0f);
+    float2* out = img + ((long long)b * fg.Ny + iy) * fg.Nx;
+    for (int ix = t; ix < fg.Nx; ix += kFftRowThreads) {
+        const int x = ix + fg.offx;
+        float2 v = sb[fft_slot<true>(x, 1)];
+        const float f = sy * fg.fx[x] * ((x & 1) ? -1.0f : 1.0f);
+        v.x *= f;
+        v.y *= f;
+        out[ix] = v;
+    }
+}
+
+// Column transforms along y for kFftColW consecutive columns of one batch.
+// BAND_IN: rows outside [offy, offy + Ny) are zero (not read).
+// BAND_OUT: only rows inside [offy, offy + Ny) are written.
+template <int OY, bool BAND_IN, bool BAND_OUT>
+__global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid, int Ox, FftGeom fg,
+                                                       const float2* __restrict__ tw, float s) {
+    constexpr int W = kFftColW;
+    constexpr int TPS = 256 / W;
+    __shared__ float2 sbuf[OY * W];
+    __shared__ float2 stw[OY];
+    load_twiddles<OY>(stw, tw);
+    const int nxb = (Ox + W - 1) / W;
+    const int b = blockIdx.x / nxb, x0 = (blockIdx.x % nxb) * W;
+    const int xl = threadIdx.x % W, yr = threadIdx.x / W;
+    const int x = x0 + xl;
+    const bool xin = x < Ox;
+    const long long rs = (long long)fg.nb * Ox;  // grid row stride
+    float2* col = grid + (long long)b * Ox + x;
+    const int ylo = fg.offy, yhi = fg.offy + fg.Ny;
+    for (int y = yr; y < OY; y += TPS) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if (xin && (!BAND_IN || (y >= ylo && y < yhi))) v = col[y * rs];
+        sbuf[y * W + xl] = v;
+    }
+    __syncthreads();
+    fft_passes<OY, 1, TPS, false>(&sbuf[xl], W, yr, stw, s);
+    if (!xin) return;
+    for (int y = yr; y < OY; y += TPS) {
+        if (BAND_OUT && (y < ylo || y >= yhi)) continue;
+        col[y * rs] = sbuf[y * W + xl];
+    }
+}
+
+}  // namespace gt

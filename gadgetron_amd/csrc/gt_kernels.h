@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void pad_deapod_kernel(const float2* __restric
     }
 }
 
-// Crop (offset (N_os - N) >> 1, NFFT.hpp:358) + deapodize + centring sign + scale.
+// Crop (offset (N_os - N) >> 1, NFFT.hpp:178) + deapodize + centring sign + scale.
 // With out_os the whole grid is processed in place (in may equal out).
 __global__ __launch_bounds__(256) void crop_deapod_kernel(const float2* in,
                                                           float2* out, long long nb,

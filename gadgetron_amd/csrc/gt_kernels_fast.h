@@ -26,14 +26,13 @@ constexpr int kCoilsPerWave = 32;
 constexpr int kSubP = 65;       // LDS floats per cell: 64 lanes + 1 (odd stride, conflict-free rows)
 constexpr int kTabDw = 16;      // dwords per 2D table entry
 constexpr int kTabKW = 7;       // weight slots per dimension in a table entry
-constexpr int kChunk = 4;       // table entries whose loads are issued together
 
-// Table entry (16 dwords):
+// C2NC table entry (16 dwords):
 //   [0] sample index j (frame * M + m)
 //   [1] packed: bits 0-7 bx (int8), 8-15 by (int8), 16-23 x slot mask, 24-31 y slot mask
-//       bx/by = local coordinate of window slot 0 (tile-local for NC2C, staged-tile-local
-//       for C2NC); bit i of a mask marks slot i as valid (inside the window and, for
-//       NC2C, inside the tile).  Weights of invalid slots are 0.
+//       bx/by = coordinate of window slot 0 in the staged tile (tile origin minus
+//       floor(radius)); bit i of a mask marks slot i as inside the window.  Weights of
+//       invalid slots are 0.
 //   [2..8] x weights of slots 0..6, [9..15] y weights of slots 0..6
 
 __device__ __forceinline__ uint32_t pack_entry(int bx, int by, uint32_t mx, uint32_t my) {
@@ -113,48 +112,55 @@ __global__ void c2nc_table2d_kernel(const float* __restrict__ p, const uint32_t*
         mx |= (wx[i] != 0.0f ? 1u : 0u) << i;
         my |= (wy[i] != 0.0f ? 1u : 0u) << i;
     }
-    const int bx = lox - (tx * g.tile[0] - fr);
-    const int by = loy - (ty * g.tile[1] - fr);
+    // 0 <= bx, by <= tile for every home sample (the plan only takes radii away
+    // from float rounding of an integer); the clamp keeps reads inside LDS regardless
+    const int bx = min(max(lox - (tx * g.tile[0] - fr), 0), g.tile[0]);
+    const int by = min(max(loy - (ty * g.tile[1] - fr), 0), g.tile[1]);
     write_entry<KW>(tab + e * kTabDw, j, pack_entry(bx, by, mx, my), wx, wy, mx, my);
 }
 
-// NC2C table: per (tile, sample) pair, the in-tile slots of the window.  For a
-// grid of at least tile + KW cells along each axis the in-tile slots of one
-// axis share a single shift, so local(slot i) = base + i.
+// NC2C table, per (8 x 8 tile, sample) pair e, in two arrays:
+//   wt[16 e + c]     weight of tile column c (c = 0..7), then of tile row c - 8
+//                    (zero where the window does not reach)
+//   hd[2 e], hd[2 e + 1]   sample index j, mask of the tile rows the window covers
+// Cells are matched by their wrapped coordinate, so windows that wrap around the
+// grid edge need no special case.
+constexpr int kTabNW = 16;  // weight floats per NC2C entry (one 64-byte line)
+constexpr int kTabTile = 8;
+
 template <int KW>
 __global__ void nc2c_table2d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list,
                                     const uint32_t* __restrict__ keys, long long n, GridGeom g,
-                                    uint32_t* __restrict__ tab) {
+                                    float* __restrict__ wt, uint32_t* __restrict__ hd) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const uint32_t j = list[e];
     const int tile = (int)(keys[e] % (uint32_t)g.ntiles_total);
     const int org[2] = {(tile % g.ntiles[0]) * g.tile[0], (tile / g.ntiles[0]) * g.tile[1]};
-    float w[2][KW];
-    int base[2];
-    uint32_t mask[2];
+    float tw[2][kTabTile];
+    uint32_t rowmask = 0;
     for (int d = 0; d < 2; d++) {
+        float w[KW];
         int lo;
-        window_raw<KW>(p[(long long)j * 2 + d], g, d, w[d], lo);
-        int shift = 0;
-        bool have = false;
-        uint32_t m = 0;
+        window_raw<KW>(p[(long long)j * 2 + d], g, d, w, lo);
+#pragma unroll
+        for (int c = 0; c < kTabTile; c++) tw[d][c] = 0.0f;
         for (int i = 0; i < KW; i++) {
-            if (w[d][i] == 0.0f) continue;
+            if (w[i] == 0.0f) continue;
             const int l = wrap_cell(lo + i, g.nos[d]) - org[d];
-            if (l >= 0 && l < g.tile[d]) {
-                if (!have) {
-                    shift = l - i;
-                    have = true;
-                }
-                m |= 1u << i;
+            if (l >= 0 && l < kTabTile) {
+                tw[d][l] = w[i];
+                if (d == 1) rowmask |= 1u << l;
             }
         }
-        base[d] = shift;
-        mask[d] = m;
     }
-    write_entry<KW>(tab + e * kTabDw, j, pack_entry(base[0], base[1], mask[0], mask[1]), w[0], w[1], mask[0],
-                    mask[1]);
+    float4* t = reinterpret_cast<float4*>(wt + e * kTabNW);
+    t[0] = make_float4(tw[0][0], tw[0][1], tw[0][2], tw[0][3]);
+    t[1] = make_float4(tw[0][4], tw[0][5], tw[0][6], tw[0][7]);
+    t[2] = make_float4(tw[1][0], tw[1][1], tw[1][2], tw[1][3]);
+    t[3] = make_float4(tw[1][4], tw[1][5], tw[1][6], tw[1][7]);
+    hd[2 * e] = j;
+    hd[2 * e + 1] = rowmask;
 }
 
 struct Entry {
@@ -183,40 +189,31 @@ __device__ __forceinline__ Entry read_entry(const uint32_t* __restrict__ t, floa
 // modulus of its length (cuNDArray_operators.cu:129-145).
 __device__ __forceinline__ long long dcw_index(int coil, uint32_t j, long long FM, long long dcw_len) {
     if (dcw_len == FM) return j;
-    return ((long long)coil * FM + j) % dcw_len;
+    const long long i = (long long)coil * FM + j;
+    return i < dcw_len ? i : i % dcw_len;
 }
 
-__global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restrict__ partials,
-                                                            float* __restrict__ grid,
-                                                            const RedItem* __restrict__ red, int F, int C,
-                                                            GridGeom g, Layout L, int sign, int accumulate) {
-    const RedItem it = red[blockIdx.x];
-    const int T0 = g.tile[0], T1 = g.tile[1];
-    const int TC = T0 * T1;
-    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
-    const int t = (int)(it.key % (uint32_t)g.ntiles_total);
-    const int ox = (t % g.ntiles[0]) * T0, oy = (t / g.ntiles[0]) * T1;
-    for (int i = threadIdx.x; i < C * TC * 2; i += blockDim.x) {
-        const int comp = i & 1;
-        const int cell = (i >> 1) % TC;
-        const int cc = (i >> 1) / TC;
-        const int lx = cell % T0, ly = cell / T0;
-        const int gx = ox + lx, gy = oy + ly;
-        if (gx >= g.nos[0] || gy >= g.nos[1]) continue;
-        float v = 0.0f;
-        for (int s = 0; s < it.count; s++)
-            v += partials[(((long long)(it.first + s) * C + cc) * TC + cell) * 2 + comp];
-        if (sign && ((gx + gy) & 1)) v = -v;
-        const long long o = 2 * ((long long)gx + L.sy * gy + L.sb * ((long long)cc * F + f)) + comp;
-        if (accumulate) v += grid[o];
-        grid[o] = v;
-    }
+// How a kernel applies dcw, uniform over the launch: 0 none, 1 one weight per
+// sample shared by every coil (a wave-uniform load), 2 per coil element.
+__device__ __forceinline__ int dcw_mode(const float* dcw, long long dcw_len, long long FM) {
+    return !dcw ? 0 : (dcw_len == FM ? 1 : 2);
 }
 
-// LDS floats of the C2NC staged tile: (T0 + KW - 1) x (T1 + KW - 1) cells plus
-// KW zeroed pad cells, read (with weight 0) by the last slots of the last row.
+// LDS floats of the C2NC staged tile: (T1 + KW - 1) rows of (T0 + KW - 1)
+// cells, then one zeroed row and one zeroed cell.  A home sample's window
+// starts at staged offset 0..T in each dimension (KW - 1 = floor(2r) >=
+// 2 floor(r)), so reading all KW x KW slots reaches at most one row and one
+// cell past the staged tile, always in slots of weight 0.
+// Rounded up to whole float4s: the work item's table entries follow it.
 __host__ __device__ inline int c2nc_lds_floats(int T0, int T1, int KW) {
-    return ((T0 + KW - 1) * (T1 + KW - 1) + KW) * kSubP;
+    return (((T1 + KW) * (T0 + KW - 1) + 1) * kSubP + 3) & ~3;
+}
+
+// C2NC work items hold at most this many entries (their table is staged in LDS).
+constexpr int kC2NCSeg = 64;
+
+__host__ __device__ inline int c2nc_lds_bytes(int KW) {
+    return (c2nc_lds_floats(8, 8, KW) + kC2NCSeg * kTabDw) * 4;
 }
 
 // ---------------------------------------------------------------------------

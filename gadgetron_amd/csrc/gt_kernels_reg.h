@@ -3,99 +3,144 @@
 // nc2c_reg2d_kernel (NFFT_H_convolve_kernel, ConvolverNC2C_standard.cuh:298-355)
 //   One wavefront owns one 8 x 8 tile of the grid for 32 coils; each lane
 //   (one coil, real or imaginary part) holds the whole tile in 64 registers.
-//   A sample's window sits at one of 15 x 15 positions relative to the tile;
-//   a jump on that (wave-uniform) position selects a block of register FMAs
-//   specialised at compile time to the in-tile cells of that position, so a
-//   sample costs one FMA per in-tile cell and no LDS traffic.  Each lane sums
-//   its cells in table order: the result is bitwise repeatable.  LDS is used
-//   once per work item, to transpose the tile for coalesced stores.
+//   A table entry carries the sample's weight for each tile column and tile
+//   row (zero outside the window) and the mask of rows the window covers, so
+//   a sample costs 8 register FMAs per covered row, with the row test on the
+//   scalar unit.  The table streams through LDS in chunks of kNK entries with
+//   coalesced loads, two chunks ahead of the gridding, and the samples of the
+//   next chunk are in flight while the current one is gridded.  Each lane
+//   sums its cells in table order: the result is bitwise repeatable.  The
+//   tile leaves through LDS, transposed for coalesced stores.
 //
 // c2nc_multi2d_kernel (NFFT_convolve_kernel, ConvolverC2NC.cuh:260-307)
 //   A workgroup of kWaves wavefronts stages one tile plus the kernel halo
-//   (32 coils, centring sign applied) in LDS with batched loads; the
-//   wavefronts then take the tile's home samples in turn and gather each
-//   window, a whole row of reads before the FMAs.
+//   (32 coils, centring sign applied) and the work item's table in LDS; each
+//   wavefront then takes a contiguous run of the tile's home samples and
+//   gathers each window with all of its reads in flight before the FMAs.
 #pragma once
 
 #include "gt_kernels_fast.h"
 
 namespace gt {
 
-constexpr int kRT = 8;      // register tile edge
-constexpr int kPosSpan = 15;  // window-origin positions per dimension: -7 .. 7
-constexpr int kWaves = 4;   // wavefronts per C2NC workgroup
+constexpr int kRT = kTabTile;  // register tile edge
+constexpr int kWaves = 4;      // wavefronts per C2NC workgroup
 
-template <int KW, int BX, int BY>
-__device__ __forceinline__ void reg_update(float (&acc)[kRT * kRT], float s, const float (&wx)[kTabKW],
-                                           const float (&wy)[kTabKW]) {
-#pragma unroll
-    for (int iy = 0; iy < KW; iy++) {
-        const int ly = BY + iy;
-        if (ly < 0 || ly >= kRT) continue;
-        const float sy = s * wy[iy];
-#pragma unroll
-        for (int ix = 0; ix < KW; ix++) {
-            const int lx = BX + ix;
-            if (lx < 0 || lx >= kRT) continue;
-            acc[ly * kRT + lx] += sy * wx[ix];
-        }
-    }
+// Wave-uniform value the compiler cannot prove uniform by itself.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+constexpr int kNK = 16;  // NC2C table entries per chunk moved through LDS
+
+// Loads chunk entries [cb, cb + kNK) of an NC2C table (entries [begin, end),
+// end > begin) into registers, one float4 of weights per lane and one head
+// dword per lane (lanes 32..63 repeat lanes 0..31).  Entries at or past `end`
+// read as j = 0 with no rows (a no-op).  The loads are unconditional (clamped
+// addresses, values selected afterwards), so every path through the caller
+// issues the same number of memory operations.
+__device__ __forceinline__ void nc2c_chunk_load(const float* __restrict__ wt, const uint32_t* __restrict__ hd,
+                                                uint32_t cb, uint32_t end, int lane, float4& w, uint32_t& h) {
+    const uint32_t e = cb + (uint32_t)(lane >> 2);
+    const float4 wv = reinterpret_cast<const float4*>(wt)[(size_t)min(e, end - 1) * 4 + (lane & 3)];
+    const bool we = e < end;
+    w = make_float4(we ? wv.x : 0.0f, we ? wv.y : 0.0f, we ? wv.z : 0.0f, we ? wv.w : 0.0f);
+    const uint32_t eh = cb + (uint32_t)((lane & 31) >> 1);
+    const uint32_t hv = hd[(size_t)min(eh, end - 1) * 2 + (lane & 1)];
+    h = eh < end ? hv : 0u;
 }
 
-#define GT_RC(i)                                                                  \
-    case (i):                                                                     \
-        reg_update<KW, (i) % kPosSpan - 7, (i) / kPosSpan - 7>(acc, s, wx, wy); \
-        break;
-#define GT_RC5(i) GT_RC(i) GT_RC((i) + 1) GT_RC((i) + 2) GT_RC((i) + 3) GT_RC((i) + 4)
-#define GT_RC25(i) GT_RC5(i) GT_RC5((i) + 5) GT_RC5((i) + 10) GT_RC5((i) + 15) GT_RC5((i) + 20)
-
-template <int KW>
-__device__ __forceinline__ void reg_update_at(int pos, float (&acc)[kRT * kRT], float s,
-                                              const float (&wx)[kTabKW], const float (&wy)[kTabKW]) {
-    switch (pos) {
-        GT_RC25(0) GT_RC25(25) GT_RC25(50) GT_RC25(75) GT_RC25(100)
-        GT_RC25(125) GT_RC25(150) GT_RC25(175) GT_RC25(200)
-        default: break;
-    }
-}
-#undef GT_RC25
-#undef GT_RC5
-#undef GT_RC
-
-template <int KW>
+// DM: how dcw is applied (dcw_mode): 0 none, 1 per sample (scalar loads),
+// 2 per coil element (loaded with the samples).
+template <int DM>
 __global__ __launch_bounds__(64) void nc2c_reg2d_kernel(
     const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
-    const uint32_t* __restrict__ tab, const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g,
-    Layout L, int sign, const float* __restrict__ dcw, long long dcw_len, int accumulate) {
+    const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
+    int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
+    int accumulate) {
     __shared__ float xp[kRT * kRT / 2 * kSubP];  // half-tile transpose buffer
+    __shared__ float4 sw[2][kNK * 4];            // weights of two table chunks
+    __shared__ uint32_t sh[2][2 * kNK];          // heads (j, row mask) of two table chunks
     const int lane = threadIdx.x;
     const WorkItem it = items[blockIdx.x];
     const int c0 = blockIdx.y * kCoilsPerWave;
     const int coil = c0 + (lane >> 1);
     const bool on = coil < C;
     const long long FM = (long long)F * M;
-    const float* sl = samples + 2 * (long long)(on ? coil : 0) * FM + (lane & 1);
+    const int lc = on ? coil : 0;  // lanes past the last coil read coil 0 and store nothing
+    const float* sl = samples + 2 * (long long)lc * FM + (lane & 1);
+    const uint32_t nch = (it.end - it.begin + kNK - 1) / kNK;
 
     float acc[kRT * kRT];
 #pragma unroll
     for (int i = 0; i < kRT * kRT; i++) acc[i] = 0.0f;
 
-    for (uint32_t e0 = it.begin; e0 < it.end; e0 += kChunk) {
-        const uint32_t ne = min((uint32_t)kChunk, it.end - e0);
-        float sv[kChunk];
+    // Pipeline: while chunk c is gridded from LDS buffer c & 1, the samples of
+    // chunk c + 1 are in flight and the table of chunk c + 2 is in registers.
+    constexpr int kDV = DM == 2 ? kNK : 1;
+    float4 wreg;
+    uint32_t hreg;
+    float svA[kNK], svB[kNK], dvA[kDV], dvB[kDV];
 #pragma unroll
-        for (int k = 0; k < kChunk; k++) {
-            const uint32_t j = tab[(size_t)(e0 + min((uint32_t)k, ne - 1)) * kTabDw];
-            sv[k] = on ? sl[2 * (long long)j] : 0.0f;
-            if (dcw && on) sv[k] *= dcw[dcw_index(coil, j, FM, dcw_len)];
+    for (int k = 0; k < kNK; k++) svA[k] = svB[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kDV; k++) dvA[k] = dvB[k] = 1.0f;
+
+    auto stage = [&](int b) {
+        sw[b][lane] = wreg;
+        if (lane < 2 * kNK) sh[b][lane] = hreg;
+    };
+    auto issue = [&](int b, float (&sv)[kNK], float (&dv)[kDV]) {
+        const int hv = (int)sh[b][lane & 31];
+#pragma unroll
+        for (int k = 0; k < kNK; k++) {
+            const uint32_t j = (uint32_t)__builtin_amdgcn_readlane(hv, 2 * k);
+            sv[k] = sl[2 * (long long)j];
+            if constexpr (DM == 2) dv[k] = dcw[dcw_index(lc, j, FM, dcw_len)];
+        }
+    };
+    auto grid_chunk = [&](int b, const float (&sv)[kNK], const float (&dv)[kDV]) {
+        const int hv = (int)sh[b][lane & 31];
+        float dk[kNK];
+#pragma unroll
+        for (int k = 0; k < kNK; k++) {
+            if constexpr (DM == 1) dk[k] = dcw[(uint32_t)__builtin_amdgcn_readlane(hv, 2 * k)];
+            else if constexpr (DM == 2) dk[k] = dv[k];
+            else dk[k] = 1.0f;
         }
 #pragma unroll
-        for (int k = 0; k < kChunk; k++) {
-            if ((uint32_t)k >= ne) break;
-            float wx[kTabKW], wy[kTabKW];
-            const Entry en = read_entry(tab + (size_t)(e0 + k) * kTabDw, wx, wy);
-            const int pos = (en.by + 7) * kPosSpan + (en.bx + 7);
-            reg_update_at<KW>(pos, acc, sv[k], wx, wy);
+        for (int k = 0; k < kNK; k++) {
+            const uint32_t rows = (uint32_t)__builtin_amdgcn_readlane(hv, 2 * k + 1);
+            const float4 x0 = sw[b][4 * k], x1 = sw[b][4 * k + 1], y0 = sw[b][4 * k + 2], y1 = sw[b][4 * k + 3];
+            const float wc[kRT] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            const float wr[kRT] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+            const float s = DM ? sv[k] * dk[k] : sv[k];
+#pragma unroll
+            for (int cy = 0; cy < kRT; cy++) {
+                if (!((rows >> cy) & 1u)) continue;
+                const float sy = s * wr[cy];
+#pragma unroll
+                for (int cx = 0; cx < kRT; cx++) acc[cy * kRT + cx] = fmaf(sy, wc[cx], acc[cy * kRT + cx]);
+            }
+        }
+    };
+
+    // The steps past the last chunk stage and issue no-op entries rather than
+    // branch, so the compiler's memory-counter waits see one steady state.
+    if (nch > 0) {
+        nc2c_chunk_load(wt, hd, it.begin, it.end, lane, wreg, hreg);
+        stage(0);
+        nc2c_chunk_load(wt, hd, it.begin + kNK, it.end, lane, wreg, hreg);
+        issue(0, svA, dvA);
+        for (uint32_t c = 0;; c += 2) {
+            stage(1);
+            nc2c_chunk_load(wt, hd, it.begin + (c + 2) * kNK, it.end, lane, wreg, hreg);
+            issue(1, svB, dvB);
+            grid_chunk(0, svA, dvA);
+            if (c + 1 >= nch) break;
+            stage(0);
+            nc2c_chunk_load(wt, hd, it.begin + (c + 3) * kNK, it.end, lane, wreg, hreg);
+            issue(0, svA, dvA);
+            grid_chunk(1, svB, dvB);
+            if (c + 2 >= nch) break;
         }
     }
 
@@ -131,101 +176,158 @@ __global__ __launch_bounds__(64) void nc2c_reg2d_kernel(
     }
 }
 
-template <int KW>
-__global__ __launch_bounds__(64 * kWaves) void c2nc_multi2d_kernel(
+// Sum of the partial tiles of split tiles, in segment order (4 interleaved
+// accumulators combined in a fixed order, so the result is repeatable).  One
+// thread per (coil, cell, re/im) element of a tile; grid (split tiles, C*128/256).
+__global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restrict__ partials,
+                                                            float* __restrict__ grid,
+                                                            const RedItem* __restrict__ red, int F, int C,
+                                                            GridGeom g, Layout L, int sign, int accumulate) {
+    const RedItem it = red[blockIdx.x];
+    const int i = blockIdx.y * blockDim.x + threadIdx.x;
+    constexpr int kTileFloats = kRT * kRT * 2;
+    if (i >= C * kTileFloats) return;
+    const int comp = i & 1, cell = (i >> 1) & (kRT * kRT - 1), cc = i / kTileFloats;
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    const int t = (int)(it.key % (uint32_t)g.ntiles_total);
+    const int gx = (t % g.ntiles[0]) * kRT + (cell & (kRT - 1));
+    const int gy = (t / g.ntiles[0]) * kRT + cell / kRT;
+    if (gx >= g.nos[0] || gy >= g.nos[1]) return;
+    const long long stride = (long long)C * kTileFloats;
+    const float* src = partials + (long long)it.first * stride + i;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    int s = 0;
+    for (; s + 4 <= it.count; s += 4) {
+        a0 += src[(s + 0) * stride];
+        a1 += src[(s + 1) * stride];
+        a2 += src[(s + 2) * stride];
+        a3 += src[(s + 3) * stride];
+    }
+    for (; s < it.count; s++) a0 += src[s * stride];
+    float v = (a0 + a1) + (a2 + a3);
+    if (sign && ((gx + gy) & 1)) v = -v;
+    const long long o = 2 * ((long long)gx + L.sy * gy + L.sb * ((long long)cc * F + f)) + comp;
+    if (accumulate) v += grid[o];
+    grid[o] = v;
+}
+
+// MODE 0: no dcw, no accumulation; 1: one dcw weight per sample, no
+// accumulation; 2: anything else (per-element dcw, accumulation).
+// LDS holds at most 3 workgroups per CU (3 waves per SIMD): the waves-per-EU
+// range lets the scheduler keep a whole window's reads in flight.
+template <int KW, int MODE>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 4))) void c2nc_multi2d_kernel(
     const float* __restrict__ grid, float* __restrict__ out, const uint32_t* __restrict__ tab,
     const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g, Layout L, int sign,
     const float* __restrict__ dcw, long long dcw_len, int accumulate) {
-    extern __shared__ float sg[];  // [E1][E0][kSubP] + pad
+    extern __shared__ float sg[];  // staged grid (c2nc_lds_floats), then the item's table
+    constexpr int E0 = kRT + KW - 1, E1 = kRT + KW - 1;
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63, wave = wave_uniform(tid >> 6);
     const WorkItem it = items[blockIdx.x];
     const int c0 = blockIdx.y * kCoilsPerWave;
-    constexpr int E0 = kRT + KW - 1, E1 = kRT + KW - 1;
     const int fr = (int)floorf(g.radius);
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     const int t = (int)(it.key % (uint32_t)g.ntiles_total);
     const int sx = (t % g.ntiles[0]) * kRT - fr, sy0 = (t / g.ntiles[0]) * kRT - fr;
+    const uint32_t ne = it.end - it.begin;  // at most kC2NCSeg
+    uint4* st = reinterpret_cast<uint4*>(sg + c2nc_lds_floats(kRT, kRT, KW));
 
-    // stage tile + halo: row r = cl * E1 + ey of E0 complex cells; threads run
-    // along x, kU rows are loaded before they are stored
+    // the item's table entries, 16 dwords each
+    for (uint32_t i = tid; i < ne * 4; i += 64 * kWaves)
+        st[i] = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + i];
+
+    // tile + halo: a thread owns one float (cell ex, re or im) of a staged row
+    // and walks the E1 rows for 4 coils; the row wrap and centring sign are
+    // worked out once per row
     {
-        constexpr int rowf = 2 * E0;
-        constexpr int rp = (64 * kWaves) / rowf;  // rows per pass
-        constexpr int nrows = kCoilsPerWave * E1;
-        constexpr int kU = 4;
-        const int rs = tid / rowf, q = tid % rowf;
-        const int ex = q >> 1, comp = q & 1;
-        const int gx = wrap_cell(sx + ex, g.nos[0]);
-        if (rs < rp) {
-            for (int r0 = rs; r0 < nrows; r0 += rp * kU) {
-                float v[kU];
-                int dst[kU];
+        const int q = tid & 31, cg = tid >> 5;  // 8 coil groups of 32 threads
+        if (q < 2 * E0) {
+            const int ex = q >> 1, comp = q & 1;
+            int gx = sx + ex;
+            gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
+            const long long cstride = 2 * L.sb * (long long)F;  // next coil
+            const float* gcol = grid + 2 * (long long)gx + comp + 2 * L.sb * (long long)f;
+            float* dcol = sg + ex * kSubP + comp;
 #pragma unroll
-                for (int u = 0; u < kU; u++) {
-                    const int r = r0 + u * rp;
-                    v[u] = 0.0f;
-                    dst[u] = -1;
-                    if (r < nrows) {
-                        const int cl = r / E1, ey = r - (r / E1) * E1;
-                        const int gy = wrap_cell(sy0 + ey, g.nos[1]);
-                        const int cc = c0 + cl;
-                        if (cc < C) {
-                            v[u] = grid[2 * ((long long)gx + L.sy * gy + L.sb * ((long long)cc * F + f)) + comp];
-                            if (sign && ((gx + gy) & 1)) v[u] = -v[u];
-                        }
-                        dst[u] = (ey * E0 + ex) * kSubP + 2 * cl + comp;
-                    }
+            for (int ey = 0; ey < E1; ey++) {
+                int gy = sy0 + ey;
+                gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
+                const float sgn = (sign && ((gx + gy) & 1)) ? -1.0f : 1.0f;
+                const float* prow = gcol + 2 * L.sy * (long long)gy;
+#pragma unroll
+                for (int u = 0; u < kCoilsPerWave / 8; u++) {
+                    const int cl = cg + 8 * u;
+                    float v = 0.0f;
+                    if (c0 + cl < C) v = prow[(long long)(c0 + cl) * cstride] * sgn;
+                    dcol[ey * E0 * kSubP + 2 * cl] = v;
                 }
-#pragma unroll
-                for (int u = 0; u < kU; u++)
-                    if (dst[u] >= 0) sg[dst[u]] = v[u];
             }
         }
-        for (int i = tid; i < KW * kSubP; i += 64 * kWaves) sg[E0 * E1 * kSubP + i] = 0.0f;
+        for (int i = tid; i < (E0 + 1) * kSubP; i += 64 * kWaves) sg[E0 * E1 * kSubP + i] = 0.0f;
     }
     __syncthreads();
 
+    // lanes past the last coil mirror coil c0 (its staged values, its output
+    // address): they store the same value to the same place, so no lane masking
     const int coil = c0 + (lane >> 1);
     const bool on = coil < C;
+    const int lc = on ? coil : c0;
     const long long FM = (long long)F * M;
-    float* ol = out + 2 * (long long)(on ? coil : 0) * FM + (lane & 1);
-    const float* sl = sg + lane;
-    for (uint32_t e0 = it.begin + (uint32_t)wave * kChunk; e0 < it.end; e0 += kChunk * kWaves) {
-        const uint32_t ne = min((uint32_t)kChunk, it.end - e0);
-        float pre[kChunk];
+    const int dm = dcw_mode(dcw, dcw_len, FM);
+    float* ol = out + 2 * (long long)lc * FM + (lane & 1);
+    const float* sl = sg + (on ? lane : (lane & 1));
+    // wave w takes a contiguous quarter of the entries (consecutive samples of
+    // a spoke share output cache lines)
+    const uint32_t q0 = (ne * (uint32_t)wave) / kWaves, q1 = (ne * (uint32_t)(wave + 1)) / kWaves;
+    // window sum of entry k; j is its (wave-uniform) sample index
+    auto gather = [&](uint32_t k, uint32_t& j) -> float {
+        const uint4 h0 = st[4 * k], h1 = st[4 * k + 1], h2 = st[4 * k + 2], h3 = st[4 * k + 3];
+        j = (uint32_t)__builtin_amdgcn_readfirstlane((int)h0.x);
+        const uint32_t pk = h0.y;
+        const int bx = (int)(int8_t)(pk & 0xff), by = (int)(int8_t)((pk >> 8) & 0xff);
+        const float wx[kTabKW] = {__uint_as_float(h0.z), __uint_as_float(h0.w), __uint_as_float(h1.x),
+                                  __uint_as_float(h1.y), __uint_as_float(h1.z), __uint_as_float(h1.w),
+                                  __uint_as_float(h2.x)};
+        const float wy[kTabKW] = {__uint_as_float(h2.y), __uint_as_float(h2.z), __uint_as_float(h2.w),
+                                  __uint_as_float(h3.x), __uint_as_float(h3.y), __uint_as_float(h3.z),
+                                  __uint_as_float(h3.w)};
+        // every slot of the window is read (slots outside it have weight 0
+        // and stay inside the staged tile and its zeroed pad), all reads first
+        const float* base = sl + (by * E0 + bx) * kSubP;
+        float v[KW][KW];
 #pragma unroll
-        for (int k = 0; k < kChunk; k++) {
-            const uint32_t j = tab[(size_t)(e0 + min((uint32_t)k, ne - 1)) * kTabDw];
-            float v = 1.0f;
-            if (dcw && on) v = dcw[dcw_index(coil, j, FM, dcw_len)];
-            pre[k] = v;
+        for (int iy = 0; iy < KW; iy++)
+#pragma unroll
+            for (int ix = 0; ix < KW; ix++) v[iy][ix] = base[(iy * E0 + ix) * kSubP];
+        float a = 0.0f;
+#pragma unroll
+        for (int iy = 0; iy < KW; iy++) {
+            float r = 0.0f;
+#pragma unroll
+            for (int ix = 0; ix < KW; ix++) r += wx[ix] * v[iy][ix];
+            a += wy[iy] * r;
         }
-#pragma unroll
-        for (int k = 0; k < kChunk; k++) {
-            if ((uint32_t)k >= ne) break;
-            float wx[kTabKW], wy[kTabKW];
-            const Entry en = read_entry(tab + (size_t)(e0 + k) * kTabDw, wx, wy);
-            const float* base = sl + (en.by * E0 + en.bx) * kSubP;
-            float a = 0.0f;
-#pragma unroll
-            for (int iy = 0; iy < KW; iy++) {
-                if (!((en.my >> iy) & 1u)) continue;
-                const float* row = base + iy * E0 * kSubP;
-                float v[KW];
-#pragma unroll
-                for (int ix = 0; ix < KW; ix++) v[ix] = row[ix * kSubP];
-                float r = 0.0f;
-#pragma unroll
-                for (int ix = 0; ix < KW; ix++) r += wx[ix] * v[ix];
-                a += wy[iy] * r;
-            }
-            if (on) {
-                const long long o = 2 * (long long)en.j;
-                a *= pre[k];
-                if (accumulate) a += ol[o];
-                ol[o] = a;
-            }
+        return a;
+    };
+    if constexpr (MODE < 2) {
+        // no vector loads in the loop (a per-sample dcw is a scalar load), so
+        // no wait ever covers the stores already issued
+        for (uint32_t k = q0; k < q1; k++) {
+            uint32_t j;
+            float a = gather(k, j);
+            if constexpr (MODE == 1) a *= dcw[j];
+            ol[2 * (long long)j] = a;
+        }
+    } else {
+        for (uint32_t k = q0; k < q1; k++) {
+            uint32_t j;
+            float a = gather(k, j);
+            if (dm == 1) a *= dcw[j];
+            else if (dm == 2) a *= dcw[dcw_index(lc, j, FM, dcw_len)];
+            const long long o = 2 * (long long)j;
+            if (accumulate) a += ol[o];
+            ol[o] = a;
         }
     }
 }

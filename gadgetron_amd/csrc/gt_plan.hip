@@ -39,6 +39,7 @@
 #include "gt_kernels.h"
 #include "gt_kernels_fast.h"
 #include "gt_kernels_reg.h"
+#include "gt_fft.h"
 
 namespace {
 
@@ -125,17 +126,41 @@ struct FFTPlan {
 
 enum FFTKind { FFT_IMAGE = 0, FFT_ROWS = 1, FFT_COLS = 2, FFT_GRID2D = 3 };
 
-// How a 2D compute transforms its row-interleaved grids: two 1-D passes that
-// skip the zero / cropped rows, or one strided 2-D rocFFT plan.  The
-// GT_NUFFT_FFT environment variable ("passes" or "grid2d") overrides the default.
+// How a 2D compute transforms its row-interleaved grids:
+//   2 (default) the fused band transforms of gt_fft.h (pad / crop and
+//     deapodization inside the row transforms), for compiled lengths;
+//   1 one strided 2-D rocFFT plan; 0 two 1-D rocFFT passes that skip the zero
+//     or cropped rows (1 and 0 with separate pad / crop kernels).
+// The GT_NUFFT_FFT environment variable ("fused", "grid2d", "passes")
+// overrides the default.
 int fft2d_strategy() {
     static int mode = [] {
         const char* e = std::getenv("GT_NUFFT_FFT");
         if (e && std::string(e) == "passes") return 0;
         if (e && std::string(e) == "grid2d") return 1;
-        return 1;
+        return 2;
     }();
     return mode;
+}
+
+// Transform lengths with compiled fused kernels.
+template <template <int> class K, class... A>
+bool dispatch_fft_len(int L, A&&... a) {
+    switch (L) {
+#define GT_LEN(l) \
+    case l: K<l>::run(std::forward<A>(a)...); return true;
+        GT_LEN(96) GT_LEN(128) GT_LEN(144) GT_LEN(160) GT_LEN(192) GT_LEN(240) GT_LEN(256) GT_LEN(288)
+        GT_LEN(320) GT_LEN(384)
+#undef GT_LEN
+    default: return false;
+    }
+}
+bool fused_fft_len(int L) {
+    switch (L) {
+    case 96: case 128: case 144: case 160: case 192: case 240: case 256: case 288: case 320: case 384:
+        return true;
+    default: return false;
+    }
 }
 
 // Sample lists of one binning: the tile-sorted sample indices, the
@@ -144,6 +169,7 @@ struct Binning {
     gt::GridGeom geom{};
     DevBuf list;      // sample index per entry, sorted by tile key then sample
     DevBuf table;     // per-entry Kaiser-Bessel table (2D hot path)
+    DevBuf heads;     // per-entry (sample index, row mask) of the NC2C table
     DevBuf items;     // gt::WorkItem
     DevBuf red;       // gt::RedItem (split tiles)
     size_t n = 0;
@@ -151,6 +177,7 @@ struct Binning {
     void release() {
         list.release();
         table.release();
+        heads.release();
         items.release();
         red.release();
         n = 0;
@@ -171,10 +198,12 @@ struct gt_nufft_plan {
     int tile_req[3] = {0, 0, 0};
     int max_items_req = 0;
     bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
+    bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
 
     gt::GridGeom geom{};  // geometry for the generic kernels
     std::vector<float> filt_host[3];
     DevBuf filt[3];
+    DevBuf twiddle[2];  // exp(-2 pi i m / N_os[d]), m < N_os[d] (fused FFT)
 
     // preprocess state
     size_t M = 0, F = 0;
@@ -406,7 +435,7 @@ gt::PadCrop padcrop(const gt_nufft_plan* p, bool pad_side, bool in_os, float sca
     for (int d = 0; d < 3; d++) {
         pc.N[d] = d < p->D ? int(p->N[d]) : 1;
         if (d < p->D) {
-            // pad offset N_os/2 - N/2 (cuNDArray_utils.cu:283-286), crop offset (N_os-N)>>1 (NFFT.hpp:358)
+            // pad offset N_os/2 - N/2 (cuNDArray_utils.cu:283-286), crop offset (N_os-N)>>1 (NFFT.hpp:178)
             pc.off[d] = pad_side ? int(p->Nos[d] / 2 - p->N[d] / 2) : int((p->Nos[d] - p->N[d]) >> 1);
             pc.filt[d] = p->filt[d].as<float>();
         } else {
@@ -504,9 +533,9 @@ struct LaunchNC2CAtomic {
 template <int KW>
 struct LaunchTablesNC2C {
     static void run(const float* pos, const uint32_t* list, const uint32_t* keys, long long n,
-                    const gt::GridGeom& g, uint32_t* tab, hipStream_t s) {
+                    const gt::GridGeom& g, float* wt, uint32_t* hd, hipStream_t s) {
         hipLaunchKernelGGL((gt::nc2c_table2d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           pos, list, keys, n, g, tab);
+                           pos, list, keys, n, g, wt, hd);
     }
 };
 
@@ -525,8 +554,11 @@ struct LaunchFastNC2C {
                     const gt::Layout& L, int sign, const float* dcw, long long dcw_len, int accumulate,
                     hipStream_t s) {
         const unsigned chunks = (unsigned)((C + gt::kCoilsPerWave - 1) / gt::kCoilsPerWave);
-        hipLaunchKernelGGL((gt::nc2c_reg2d_kernel<KW>), dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s,
-                           samples, grid, partials, p->bn.table.as<uint32_t>(), p->bn.items.as<gt::WorkItem>(),
+        const long long FM = (long long)(p->M * p->F);
+        auto k = !dcw ? gt::nc2c_reg2d_kernel<0> : (dcw_len == FM ? gt::nc2c_reg2d_kernel<1> : gt::nc2c_reg2d_kernel<2>);
+        hipLaunchKernelGGL(k, dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s,
+                           samples, grid, partials, p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(),
+                           p->bn.items.as<gt::WorkItem>(),
                            int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw, dcw_len, accumulate);
     }
 };
@@ -537,9 +569,11 @@ struct LaunchFastC2NC {
                     const float* dcw, long long dcw_len, int accumulate, hipStream_t s) {
         const gt::GridGeom& g = p->bc.geom;
         const unsigned chunks = (unsigned)((C + gt::kCoilsPerWave - 1) / gt::kCoilsPerWave);
-        const size_t lds = size_t(gt::c2nc_lds_floats(g.tile[0], g.tile[1], KW)) * sizeof(float);
-        hipLaunchKernelGGL((gt::c2nc_multi2d_kernel<KW>), dim3((unsigned)p->bc.n_items, chunks),
-                           dim3(64 * gt::kWaves), lds, s,
+        const size_t lds = size_t(gt::c2nc_lds_bytes(KW));
+        const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
+        auto k = mode == 0 ? gt::c2nc_multi2d_kernel<KW, 0>
+                           : (mode == 1 ? gt::c2nc_multi2d_kernel<KW, 1> : gt::c2nc_multi2d_kernel<KW, 2>);
+        hipLaunchKernelGGL(k, dim3((unsigned)p->bc.n_items, chunks), dim3(64 * gt::kWaves), lds, s,
                            grid, out, p->bc.table.as<uint32_t>(), p->bc.items.as<gt::WorkItem>(), int(p->M),
                            int(p->F), C, g, L, sign, dcw, dcw_len, accumulate);
     }
@@ -599,7 +633,12 @@ void setup_geometry(gt_nufft_plan* p) {
             ok &= ft[d] == gt::kRT;
             ok &= int(p->Nos[d]) >= ft[d] + p->KW;
         }
-        ok &= size_t(gt::c2nc_lds_floats(gt::kRT, gt::kRT, p->KW)) * sizeof(float) <= 65536;
+        ok &= gt::c2nc_lds_bytes(p->KW) <= 65536;
+        // window origins are found with ceil(p - r) in float: keep r clear of
+        // an integer by more than the rounding of p (or exactly on one)
+        const double fr = double(p->radius) - std::floor(double(p->radius));
+        const double eps = 4.0 * std::ldexp(1.0, -23) * double(std::max(p->Nos[0], p->Nos[1]));
+        ok &= fr == 0.0 || (fr > eps && fr < 1.0 - eps);
         if (ok) {
             p->fast2d = true;
             p->bn.geom = make_geom(p, ft);
@@ -740,8 +779,13 @@ void make_items(gt_nufft_plan* p, Binning& b, const std::vector<uint32_t>& count
 // partial tile write and read, splitting a C2NC tile one more halo staging.
 size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& counts, size_t npairs,
                       bool nc2c) {
+    if (p->fast2d && !nc2c) {
+        // the 2D C2NC kernel stages a work item's whole table in LDS
+        const size_t cap = size_t(gt::kC2NCSeg);
+        return p->max_items_req > 0 ? std::min(cap, size_t(p->max_items_req)) : cap;
+    }
     if (p->max_items_req > 0) return size_t(p->max_items_req);
-    if (p->fast2d) return nc2c ? 256 : 128;
+    if (p->fast2d) return 256;
     const double avg = double(npairs) / double(std::max<size_t>(counts.size(), 1));
     return size_t(std::max(256.0, 4.0 * avg));
 }
@@ -751,11 +795,12 @@ void build_nc2c(gt_nufft_plan* p, hipStream_t s) {
     if (p->fast2d) {
         DevBuf keys_sorted;
         auto counts = bin_windows(p, b.geom, b, &keys_sorted, s);
-        b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
+        b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabNW * sizeof(float));
+        b.heads.reserve(std::max<size_t>(b.n, 1) * 2 * sizeof(uint32_t));
         if (b.n)
             dispatch_fast<LaunchTablesNC2C>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(),
                                             keys_sorted.as<uint32_t>(), (long long)b.n, b.geom,
-                                            b.table.as<uint32_t>(), s);
+                                            b.table.as<float>(), b.heads.as<uint32_t>(), s);
         GT_HIP(hipGetLastError());
         make_items(p, b, counts, segment_length(p, counts, b.n, true), true, s);
         keys_sorted.release();
@@ -853,7 +898,8 @@ void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layo
                                       int(C), L, sign, dcw, dl, accumulate ? 1 : 0, s);
         GT_HIP(hipGetLastError());
         if (b.n_red > 0) {
-            hipLaunchKernelGGL(gt::nc2c_reduce2d_kernel, dim3((unsigned)b.n_red), dim3(256), 0, s,
+            const unsigned parts = (unsigned)((C * gt::kRT * gt::kRT * 2 + 255) / 256);
+            hipLaunchKernelGGL(gt::nc2c_reduce2d_kernel, dim3((unsigned)b.n_red, parts), dim3(256), 0, s,
                                p->partials.as<float>(), (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
                                b.geom, L, sign, accumulate ? 1 : 0);
             GT_HIP(hipGetLastError());
@@ -910,9 +956,102 @@ gt::RowGeom row_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float sca
     return rg;
 }
 
+// ---- fused band FFT launchers (gt_fft.h) -------------------------------------
+template <int L>
+struct LaunchRowsPad {
+    static void run(const float2* img, float2* grid, const gt::FftGeom& fg, const float2* tw, float dir,
+                    hipStream_t s) {
+        const unsigned blocks = unsigned((size_t(fg.nb) * fg.Ny + gt::kFftRows - 1) / gt::kFftRows);
+        hipLaunchKernelGGL(gt::fft_rows_pad_kernel<L>, dim3(blocks), dim3(gt::kFftRowThreads * gt::kFftRows), 0, s,
+                           img, grid, fg, tw, dir);
+    }
+};
+template <int L>
+struct LaunchRowsCrop {
+    static void run(const float2* grid, float2* img, const gt::FftGeom& fg, const float2* tw, float dir,
+                    hipStream_t s) {
+        const unsigned blocks = unsigned((size_t(fg.nb) * fg.Ny + gt::kFftRows - 1) / gt::kFftRows);
+        hipLaunchKernelGGL(gt::fft_rows_crop_kernel<L>, dim3(blocks), dim3(gt::kFftRowThreads * gt::kFftRows), 0,
+                           s, grid, img, fg, tw, dir);
+    }
+};
+template <int L>
+struct LaunchColsBandIn {
+    static void run(float2* grid, int Ox, const gt::FftGeom& fg, const float2* tw, float dir, hipStream_t s) {
+        const unsigned blocks = unsigned(size_t(fg.nb) * ((Ox + gt::kFftColW - 1) / gt::kFftColW));
+        hipLaunchKernelGGL((gt::fft_cols_kernel<L, true, false>), dim3(blocks), dim3(256), 0, s, grid, Ox, fg, tw,
+                           dir);
+    }
+};
+template <int L>
+struct LaunchColsBandOut {
+    static void run(float2* grid, int Ox, const gt::FftGeom& fg, const float2* tw, float dir, hipStream_t s) {
+        const unsigned blocks = unsigned(size_t(fg.nb) * ((Ox + gt::kFftColW - 1) / gt::kFftColW));
+        hipLaunchKernelGGL((gt::fft_cols_kernel<L, false, true>), dim3(blocks), dim3(256), 0, s, grid, Ox, fg, tw,
+                           dir);
+    }
+};
+
+gt::FftGeom fft_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float scale) {
+    const gt::RowGeom rg = row_geom(p, nb, pad_side, scale);
+    gt::FftGeom fg{};
+    fg.Nx = rg.Nx;
+    fg.Ny = rg.Ny;
+    fg.offx = rg.offx;
+    fg.offy = rg.offy;
+    fg.nb = rg.nb;
+    fg.fx = rg.fx;
+    fg.fy = rg.fy;
+    fg.scale = rg.scale;
+    return fg;
+}
+
+// 2D hot path with the fused band transforms: pad + deapodize + row FFT over
+// the image rows, then column FFTs (C2NC side); column FFTs writing only the
+// cropped rows, then row FFT + crop + deapodize (NC2C side).
+void compute_fast2d_fused(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const float* dcw,
+                          size_t dcw_len, int mode, float scale, hipStream_t s) {
+    const size_t nb = sh.nb;
+    const int Ox = int(p->Nos[0]), Oy = int(p->Nos[1]);
+    GT_REQUIRE(nb * p->N[1] < (size_t(1) << 31) && nb * size_t((Ox + 15) / 16) < (size_t(1) << 31),
+               GT_ERR_UNSUPPORTED, "too many batches for one launch");
+    p->grid.reserve(nb * p->G() * sizeof(float2));
+    float2* g = p->grid.as<float2>();
+    const gt::Layout L = internal_layout(p, nb);
+    const float2* twx = p->twiddle[0].as<float2>();
+    const float2* twy = p->twiddle[1].as<float2>();
+    if (mode == GT_FORWARDS_C2NC || mode == GT_BACKWARDS_C2NC) {
+        const float dir = mode == GT_FORWARDS_C2NC ? -1.0f : 1.0f;
+        const gt::FftGeom fg = fft_geom(p, nb, true, scale);
+        {
+            StageTimer timer(p, GT_STAGE_FFT, s);
+            dispatch_fft_len<LaunchRowsPad>(Ox, (const float2*)in, g, fg, twx, dir, s);
+            GT_HIP(hipGetLastError());
+            dispatch_fft_len<LaunchColsBandIn>(Oy, g, Ox, fg, twy, dir, s);
+            GT_HIP(hipGetLastError());
+        }
+        conv_c2nc(p, g, L, out, sh.C, 1, dcw, dcw_len, false, s);
+    } else {
+        const float dir = mode == GT_FORWARDS_NC2C ? -1.0f : 1.0f;
+        const gt::FftGeom fg = fft_geom(p, nb, false, scale);
+        conv_nc2c(p, in, g, L, sh.C, 1, dcw, dcw_len, false, s);
+        {
+            StageTimer timer(p, GT_STAGE_FFT, s);
+            dispatch_fft_len<LaunchColsBandOut>(Oy, g, Ox, fg, twy, dir, s);
+            GT_HIP(hipGetLastError());
+            dispatch_fft_len<LaunchRowsCrop>(Ox, (const float2*)g, (float2*)out, fg, twx, dir, s);
+            GT_HIP(hipGetLastError());
+        }
+    }
+}
+
 // 2D hot path, image side of size N: internal layout [y][batch][x].
 void compute_fast2d(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const float* dcw, size_t dcw_len,
                     int mode, float scale, hipStream_t s) {
+    if (p->fft_fused && fft2d_strategy() == 2) {
+        compute_fast2d_fused(p, in, out, sh, dcw, dcw_len, mode, scale, s);
+        return;
+    }
     const size_t nb = sh.nb;
     const size_t Ox = p->Nos[0];
     p->grid.reserve(nb * p->G() * sizeof(float2));
@@ -974,7 +1113,7 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
     set_device(p);
     const size_t G = p->G();
     const float gs = centring_global_sign(p);
-    const float scale = gs / std::sqrt(float(G));  // do_scale=true, NFFT.hpp:386-418
+    const float scale = gs / std::sqrt(float(G));  // do_scale=true (NFFT.h:114-115, NFFT.hpp:205-238)
     if (p->fast2d && !sh.oversampled) {
         compute_fast2d(p, in, out, sh, dcw, dcw_len, mode, scale, s);
         return;
@@ -983,7 +1122,7 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
     switch (mode) {
     case GT_FORWARDS_C2NC:
     case GT_BACKWARDS_C2NC: {
-        // FORWARDS_C2NC: deapodize(image) -> FFT forwards -> C2NC (NFFT.hpp:386-391).
+        // FORWARDS_C2NC: deapodize(image) -> FFT forwards -> C2NC (NFFT.hpp:205-211).
         // BACKWARDS_C2NC: the adjoint of FORWARDS_NC2C (deapodize(fourier) -> FFT backwards -> C2NC).
         p->grid.reserve(sh.nb * G * sizeof(float2));
         pad_deapod(p, in, p->grid.ptr, sh.nb, sh.oversampled, scale, s);
@@ -993,8 +1132,8 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
     }
     case GT_BACKWARDS_NC2C:
     case GT_FORWARDS_NC2C: {
-        // BACKWARDS_NC2C: dcw -> NC2C -> FFT backwards -> deapodize -> crop (NFFT.hpp:403-409).
-        // FORWARDS_NC2C: dcw -> NC2C -> FFT forwards -> deapodize(fourier) -> crop (NFFT.hpp:394-400).
+        // BACKWARDS_NC2C: dcw -> NC2C -> FFT backwards -> deapodize -> crop (NFFT.hpp:223-229).
+        // FORWARDS_NC2C: dcw -> NC2C -> FFT forwards -> deapodize(fourier) -> crop (NFFT.hpp:214-220).
         void* g = out;
         if (!sh.oversampled) {
             p->grid.reserve(sh.nb * G * sizeof(float2));
@@ -1042,7 +1181,7 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
             p->N[i] = d->N[i];
             float osf;
             if (d->os_factor != 0.0) {
-                // NFFT.hpp:203-217: N_os = size_t(alpha * N); KaiserKernel os ctor (ConvolutionKernel.hpp:241-254)
+                // NFFT.hpp:23-37: N_os = size_t(alpha * N); KaiserKernel os ctor (ConvolutionKernel.hpp:241-254)
                 const float a = float(d->os_factor);
                 p->Nos[i] = size_t(a * float(d->N[i]));
                 osf = a;
@@ -1074,6 +1213,20 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         }
         p->flag.reserve(sizeof(int));
         setup_geometry(p.get());
+        if (p->D == 2 && fused_fft_len(int(p->Nos[0])) && fused_fft_len(int(p->Nos[1]))) {
+            p->fft_fused = true;
+            for (int i = 0; i < 2; i++) {
+                const size_t L = p->Nos[i];
+                std::vector<float> tw(2 * L);
+                for (size_t m = 0; m < L; m++) {
+                    const double a = -2.0 * M_PI * double(m) / double(L);
+                    tw[2 * m] = float(std::cos(a));
+                    tw[2 * m + 1] = float(std::sin(a));
+                }
+                p->twiddle[i].reserve(L * sizeof(float2));
+                GT_HIP(hipMemcpy(p->twiddle[i].ptr, tw.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+            }
+        }
         *out = p.release();
     });
 }

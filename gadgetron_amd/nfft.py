@@ -110,6 +110,10 @@ class DeviceArray:
 def _ptr(a):
     if a is None:
         return None
+    # the ABI takes dense x-fastest arrays: a strided view (e.g. a transposed
+    # torch tensor) would be read with the wrong strides
+    if hasattr(a, "is_contiguous") and not a.is_contiguous():
+        raise GTError(3, "array must be contiguous")
     return ctypes.c_void_p(int(a.data_ptr()))
 
 

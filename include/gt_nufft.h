@@ -61,8 +61,8 @@ typedef void* gt_stream_t; /* a hipStream_t; NULL selects the default stream */
 typedef struct gt_nufft_plan gt_nufft_plan;
 
 /* Plan descriptor.  One of N_os[] or os_factor selects the ctor:
- *   os_factor == 0  -> NFFT_plan(N, N_os, W)              (NFFT.hpp:186-200, cuNFFT.cu:28-47)
- *   os_factor >= 1  -> NFFT_plan(N, os_factor, W)         (NFFT.hpp:203-217)
+ *   os_factor == 0  -> NFFT_plan(N, N_os, W)              (NFFT.hpp:6-20, cuNFFT.cu:28-47)
+ *   os_factor >= 1  -> NFFT_plan(N, os_factor, W)         (NFFT.hpp:23-37)
  * Both use the cuNFFT deapodization convention (unscaled filter FFT,
  * cuNFFT.cu:170-202), which equals hoNFFT_plan(N, alpha, W) (hoNFFT.cpp:175-197). */
 typedef struct gt_nufft_desc {
@@ -80,21 +80,21 @@ typedef struct gt_nufft_desc {
 gt_status gt_nufft_create(gt_nufft_plan** plan, const gt_nufft_desc* desc);
 gt_status gt_nufft_destroy(gt_nufft_plan* plan);
 
-/* NFFT_plan::preprocess (NFFT.hpp:220-242, cuGriddingConvolution.cu:128-188).
+/* NFFT_plan::preprocess (NFFT.hpp:40-62, cuGriddingConvolution.cu:128-188).
  * traj: device pointer to num_samples*num_frames D-vectors of float. */
 gt_status gt_nufft_preprocess(gt_nufft_plan* plan, const void* traj, size_t num_samples,
                               size_t num_frames, int prep_mode, gt_stream_t stream);
 
-/* NFFT_plan::compute (NFFT.hpp:284-382).  dcw may be NULL. */
+/* NFFT_plan::compute (NFFT.hpp:104-202).  dcw may be NULL. */
 gt_status gt_nufft_compute(gt_nufft_plan* plan, const void* in, const size_t* in_dims, int in_ndim,
                            void* out, const size_t* out_dims, int out_ndim, const float* dcw,
                            size_t dcw_len, int comp_mode, gt_stream_t stream);
 
-/* NFFT_plan::mult_MH_M (NFFT.hpp:269-281): FORWARDS_C2NC then BACKWARDS_NC2C. */
+/* NFFT_plan::mult_MH_M (NFFT.hpp:89-101): FORWARDS_C2NC then BACKWARDS_NC2C. */
 gt_status gt_nufft_mult_MH_M(gt_nufft_plan* plan, const void* in, void* out, const size_t* dims,
                              int ndim, const float* dcw, size_t dcw_len, gt_stream_t stream);
 
-/* NFFT_plan::convolve (NFFT.hpp:245-266); the image side is oversampled. */
+/* NFFT_plan::convolve (NFFT.hpp:65-86); the image side is oversampled. */
 gt_status gt_nufft_convolve(gt_nufft_plan* plan, const void* in, const size_t* in_dims, int in_ndim,
                             void* out, const size_t* out_dims, int out_ndim, int conv_mode,
                             int accumulate, gt_stream_t stream);

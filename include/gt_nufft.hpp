@@ -96,7 +96,7 @@ class cuNFFT_plan {
 public:
     using vec = std::array<size_t, D>;
 
-    // NFFT_plan(N, N_os, W) (NFFT.hpp:186-200)
+    // NFFT_plan(N, N_os, W) (NFFT.hpp:6-20)
     cuNFFT_plan(const vec& matrix_size, const vec& matrix_size_os, REAL W,
                 ConvolutionType conv = ConvolutionType::STANDARD, int device = -1) {
         gt_nufft_desc d{};
@@ -104,7 +104,7 @@ public:
         for (unsigned i = 0; i < D; i++) d.N_os[i] = matrix_size_os[i];
         create(d);
     }
-    // NFFT_plan(N, os_factor, W) (NFFT.hpp:203-217)
+    // NFFT_plan(N, os_factor, W) (NFFT.hpp:23-37)
     cuNFFT_plan(const vec& matrix_size, REAL os_factor, REAL W,
                 ConvolutionType conv = ConvolutionType::STANDARD, int device = -1) {
         gt_nufft_desc d{};

@@ -26,10 +26,11 @@
 //     (decimation in time, twiddles from double precision), checked against a
 //     brute-force DFT, numpy.fft and Parseval.
 //   * hoNFFT_plan::mult_MH_M reads sample/frame counts nothing assigns
-//     (hoNFFT.cpp:243-244); this restatement uses NFFT.hpp:270-281 instead.
+//     (hoNFFT.cpp:243-244); this restatement uses NFFT.hpp:89-101 instead.
 //   * Re-preprocessing clears the per-frame matrices (hoGriddingConvolution.cpp:58-72
 //     appends without clearing).
-//   * BACKWARDS_C2NC in NFFT.hpp:412-418 calls convolve(samples, image, NC2C); this
+//   * BACKWARDS_C2NC (NFFT.hpp:186-200) goes through compute_NFFTH_C2NC
+//     (NFFT.hpp:232-238), which calls convolve(samples, image, NC2C); this
 //     restatement computes the true adjoint of FORWARDS_NC2C instead.
 
 #include <algorithm>
@@ -527,7 +528,7 @@ struct Plan : PlanBase {
                 out[b * g + dst] = in[b * n + idx];
             }
     }
-    // NFFT.hpp:335-336,358: crop offset (N_os - N) >> 1
+    // NFFT.hpp:155,178: crop offset (N_os - N) >> 1
     void crop(const C* in, C* out, size_t nbatches) const {
         const size_t g = G(), n = Nimg();
         size_t off[3];
@@ -550,7 +551,7 @@ struct Plan : PlanBase {
         for (size_t i = 0; i < n; i++) s[i] *= dcw[i % dcw_len];
     }
 
-    // NFFT.hpp:284-382
+    // NFFT.hpp:104-202
     void compute(const C* in, const std::vector<size_t>& in_dims, C* out,
                  const std::vector<size_t>& out_dims, const R* dcw, size_t dcw_len, int mode) {
         if (cm.empty()) throw std::runtime_error("compute: plan not preprocessed");
@@ -628,7 +629,7 @@ std::unique_ptr<PlanBase> make_plan(int D, int precision, const size_t* N, const
         for (int d = 0; d < D; d++) {
             p->N[d] = N[d];
             if (ctor == 1) {
-                // NFFT.hpp:203-217 / ConvolutionKernel.hpp:241-254
+                // NFFT.hpp:23-37 / ConvolutionKernel.hpp:241-254
                 R a = R(alpha);
                 p->Nos[d] = size_t(a * R(N[d]));
                 p->kernel.os_factor[d] = a;
@@ -747,7 +748,7 @@ int gto_compute(gto_plan* h, const void* in, const size_t* in_dims, int in_ndim,
     });
 }
 
-// NFFT.hpp:269-281: FORWARDS_C2NC into [M, F, batches], then BACKWARDS_NC2C.
+// NFFT.hpp:89-101: FORWARDS_C2NC into [M, F, batches], then BACKWARDS_NC2C.
 int gto_mult_MH_M(gto_plan* h, const void* in, const size_t* dims, int ndim, void* out,
                   const void* dcw, size_t dcw_len) {
     return guarded([&] {
@@ -771,7 +772,7 @@ int gto_mult_MH_M(gto_plan* h, const void* in, const size_t* dims, int ndim, voi
     });
 }
 
-// NFFT.hpp:245-266 (standalone convolution on an oversampled image)
+// NFFT.hpp:65-86 (standalone convolution on an oversampled image)
 int gto_convolve(gto_plan* h, const void* in, void* out, size_t nbatches, int conv_mode,
                  int accumulate) {
     return guarded([&] {

@@ -182,7 +182,7 @@ def test_nfft_adjoint_identity_and_mult_MH_M():
     Ax = p.compute(x, (C, F, M), po.FORWARDS_C2NC, dcw=dcw)
     AHy = p.compute(y, (C, F, 12, 16), po.BACKWARDS_NC2C, dcw=dcw)
     assert abs(np.vdot(y, Ax) - np.vdot(AHy, x)) / abs(np.vdot(y, Ax)) < 1e-12
-    # mult_MH_M = BACKWARDS_NC2C(FORWARDS_C2NC(x)) (NFFT.hpp:269-281)
+    # mult_MH_M = BACKWARDS_NC2C(FORWARDS_C2NC(x)) (NFFT.hpp:89-101)
     mhm = p.mult_MH_M(x, dcw=dcw)
     ref = p.compute(p.compute(x, (C, F, M), po.FORWARDS_C2NC, dcw=dcw), x.shape, po.BACKWARDS_NC2C, dcw=dcw)
     assert rel_l2(mhm, ref) < 1e-14
