@@ -14,9 +14,11 @@ from .nfft import (
     NFFTPlan,
     device_count,
 )
+from .sense import CgBatch, SenseOperator, cg_sense
 from . import trajectories
 
 __all__ = [
-    "ConvolutionType", "DeviceArray", "EXPORTED_SYMBOLS", "GTError", "LIB_PATH", "NFFT_comp_mode",
-    "NFFT_conv_mode", "NFFT_fft_mode", "NFFT_prep_mode", "NFFTPlan", "device_count", "lib", "trajectories",
+    "CgBatch", "ConvolutionType", "DeviceArray", "EXPORTED_SYMBOLS", "GTError", "LIB_PATH", "NFFT_comp_mode",
+    "NFFT_conv_mode", "NFFT_fft_mode", "NFFT_prep_mode", "NFFTPlan", "SenseOperator", "cg_sense",
+    "device_count", "lib", "trajectories",
 ]

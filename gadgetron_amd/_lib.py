@@ -1,4 +1,5 @@
-"""ctypes binding of libgtnufft.so (the C ABI declared in include/gt_nufft.h).
+"""ctypes binding of libgtnufft.so (the C ABI declared in include/gt_nufft.h
+and include/gt_cgsense.h).
 
 The library is built in-tree by gadgetron_amd/csrc/Makefile.  Loading fails
 loudly when it is missing: there is no CPU fallback for any entry point.
@@ -37,6 +38,16 @@ EXPORTED_SYMBOLS = [
     "gt_memset",
     "gt_stream_synchronize",
     "gt_device_synchronize",
+    # include/gt_cgsense.h
+    "gt_sense_create",
+    "gt_sense_destroy",
+    "gt_sense_mult_M",
+    "gt_sense_mult_MH",
+    "gt_sense_mult_MH_M",
+    "gt_cgsense_solve",
+    "gt_cgsense_batch_create",
+    "gt_cgsense_batch_run",
+    "gt_cgsense_batch_destroy",
 ]
 
 # gt_status
@@ -74,6 +85,17 @@ class gt_nufft_desc(ctypes.Structure):
         ("W", ctypes.c_double),
         ("conv_type", ctypes.c_int),
         ("device", ctypes.c_int),
+    ]
+
+
+class gt_cg_desc(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("tc_tolerance", ctypes.c_double),
+        ("encoding_weight", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("precond", ctypes.c_void_p),
+        ("precond_len", ctypes.c_size_t),
     ]
 
 
@@ -120,6 +142,15 @@ def lib():
         "gt_memset": [P, I, S, P],
         "gt_stream_synchronize": [P],
         "gt_device_synchronize": [],
+        "gt_sense_create": [PP, P, P, S, P, S],
+        "gt_sense_destroy": [P],
+        "gt_sense_mult_M": [P, P, P, I, P],
+        "gt_sense_mult_MH": [P, P, P, I, P],
+        "gt_sense_mult_MH_M": [P, P, P, I, P],
+        "gt_cgsense_solve": [P, P, P, ctypes.POINTER(gt_cg_desc), ctypes.POINTER(D), P],
+        "gt_cgsense_batch_create": [PP, PP, PP, PP, I, ctypes.POINTER(gt_cg_desc), I, P],
+        "gt_cgsense_batch_run": [P, P],
+        "gt_cgsense_batch_destroy": [P],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

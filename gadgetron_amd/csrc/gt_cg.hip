@@ -1,0 +1,630 @@
+// gt_cg.hip -- the non-Cartesian SENSE operator and its CG solver over the
+// NFFT plan (include/gt_cgsense.h).
+//
+// SENSE (cuNonCartesianSenseOperator.cu:14-62, sense_utilities.cu:7-123):
+// mult_M multiplies the image by every coil map and runs FORWARDS_C2NC on the
+// [N..., F, C] stack; mult_MH runs BACKWARDS_NC2C and sums conj(csm) times the
+// coil images.
+//
+// CG (cgSolver.h:219-390): the reference reads every dot product back to the
+// host (cuBLAS cdotc) and forms alpha, beta there.  Here the dot products are
+// FP64 block reductions finished on the device by one-block kernels that also
+// form the step sizes and the termination flag, so an iteration issues no
+// host round trip and a whole solve can be captured into a hipGraph.
+// Reductions run in a fixed order: results repeat bit for bit, and the plain
+// and batched solvers agree exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "../../include/gt_cgsense.h"
+#include "gt_common.h"
+
+using namespace gt_host;
+
+namespace {
+
+constexpr int kRedBlocks = 512;   // partial sums per FP64 reduction
+constexpr int kRedThreads = 256;
+
+// ---------------------------------------------------------------------------
+// coil kernels (sense_utilities.cu:7-123)
+// ---------------------------------------------------------------------------
+// out[n, f, c] = in[n, f] * csm[n, c]
+__global__ void csm_mult_kernel(const float2* __restrict__ in, float2* __restrict__ out,
+                                const float2* __restrict__ csm, long long nimg, int F, int C) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (n >= nimg) return;
+    const float2 v = in[n + nimg * f];
+    for (int c = 0; c < C; c++) {
+        const float2 s = csm[n + nimg * c];
+        out[n + nimg * (f + (long long)F * c)] = make_float2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
+    }
+}
+
+// out[n, f] (+)= sum_c conj(csm[n, c]) * in[n, f, c]
+__global__ void csm_conj_sum_kernel(const float2* __restrict__ in, float2* __restrict__ out,
+                                    const float2* __restrict__ csm, long long nimg, int F, int C, int accumulate) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (n >= nimg) return;
+    float2 a = make_float2(0.0f, 0.0f);
+    for (int c = 0; c < C; c++) {
+        const float2 s = csm[n + nimg * c];
+        const float2 v = in[n + nimg * (f + (long long)F * c)];
+        a.x += s.x * v.x + s.y * v.y;
+        a.y += s.x * v.y - s.y * v.x;
+    }
+    float2* o = out + n + nimg * f;
+    if (accumulate) {
+        a.x += o->x;
+        a.y += o->y;
+    }
+    *o = a;
+}
+
+__global__ void add_kernel(float2* __restrict__ out, const float2* __restrict__ in, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2(out[i].x + in[i].x, out[i].y + in[i].y);
+}
+
+// ---------------------------------------------------------------------------
+// CG state and FP64 reductions
+// ---------------------------------------------------------------------------
+struct CgState {
+    double rq0, rq;
+    double alpha_re, alpha_im;
+    double beta;
+    int done;  // set after the iteration where rq / rq0 < tolerance
+    int it;
+};
+
+struct CgParams {
+    long long n;            // image elements
+    float we_re;            // encoding operator weight
+    float kappa;            // identity regularisation weight
+    const float2* w;        // preconditioner weights or null
+    long long wlen;
+    double tol;
+};
+
+// Block reduction of a double2 in a fixed order.
+__device__ __forceinline__ double2 block_sum(double2 v, double2* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[threadIdx.x].x += sh[threadIdx.x + s].x;
+            sh[threadIdx.x].y += sh[threadIdx.x + s].y;
+        }
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+__device__ __forceinline__ float2 precond2(const CgParams& P, long long i, float2 v) {
+    // cgPreconditioner::apply twice: v * w * w (weights broadcast by modulus)
+    const float2 w = P.w[i % P.wlen];
+    const float2 t = make_float2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+    return make_float2(t.x * w.x - t.y * w.y, t.x * w.y + t.y * w.x);
+}
+
+// Initialisation: r = w_E * mhd; p = r (or P^2 r); partial sums of <r, p>.
+__global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __restrict__ mhd, float2* __restrict__ x,
+                                                              float2* __restrict__ r, float2* __restrict__ p,
+                                                              CgParams P, double2* __restrict__ partial) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < P.n; i += (long long)gridDim.x * kRedThreads) {
+        const float2 d = mhd[i];
+        const float2 rv = make_float2(P.we_re * d.x, P.we_re * d.y);
+        const float2 pv = P.w ? precond2(P, i, rv) : rv;
+        r[i] = rv;
+        p[i] = pv;
+        x[i] = make_float2(0.0f, 0.0f);
+        acc.x += (double)rv.x * pv.x + (double)rv.y * pv.y;  // Re(conj(r) p)
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_init_finish_kernel(const double2* __restrict__ partial, int nb,
+                                                                     CgState* st, double* hist) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) {
+        acc.x += partial[i].x;
+        acc.y += partial[i].y;
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        st->rq0 = s.x;
+        st->rq = s.x;
+        st->alpha_re = st->alpha_im = 0.0;
+        st->beta = 0.0;
+        st->done = !(s.x > 0.0);  // a zero right-hand side has the solution x = 0
+        st->it = 0;
+        if (hist) hist[0] = s.x;
+    }
+}
+
+// q = w_E q + kappa p; partial sums of <p, q>.
+__global__ __launch_bounds__(kRedThreads) void cg_q_kernel(const float2* __restrict__ p, float2* __restrict__ q,
+                                                           CgParams P, double2* __restrict__ partial) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < P.n; i += (long long)gridDim.x * kRedThreads) {
+        const float2 pv = p[i];
+        float2 qv = q[i];
+        qv = make_float2(P.we_re * qv.x + P.kappa * pv.x, P.we_re * qv.y + P.kappa * pv.y);
+        q[i] = qv;
+        acc.x += (double)pv.x * qv.x + (double)pv.y * qv.y;   // Re(conj(p) q)
+        acc.y += (double)pv.x * qv.y - (double)pv.y * qv.x;   // Im(conj(p) q)
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// alpha = rq / <p, q> (complex); zero once the solver has terminated.
+__global__ __launch_bounds__(kRedThreads) void cg_alpha_kernel(const double2* __restrict__ partial, int nb, CgState* st) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) {
+        acc.x += partial[i].x;
+        acc.y += partial[i].y;
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        const double m = s.x * s.x + s.y * s.y;
+        if (st->done || !(m > 0.0)) {
+            st->alpha_re = st->alpha_im = 0.0;
+        } else {
+            st->alpha_re = st->rq * s.x / m;
+            st->alpha_im = -st->rq * s.y / m;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; then z = P^2 r into q (or z = r); partial sums of Re<r, z>.
+__global__ __launch_bounds__(kRedThreads) void cg_xr_kernel(float2* __restrict__ x, float2* __restrict__ r,
+                                                            const float2* __restrict__ p, float2* __restrict__ q,
+                                                            CgParams P, const CgState* __restrict__ st,
+                                                            double2* __restrict__ partial) {
+    __shared__ double2 sh[kRedThreads];
+    const float ar = (float)st->alpha_re, ai = (float)st->alpha_im;
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < P.n; i += (long long)gridDim.x * kRedThreads) {
+        const float2 pv = p[i], qv = q[i];
+        float2 xv = x[i], rv = r[i];
+        xv.x += ar * pv.x - ai * pv.y;
+        xv.y += ar * pv.y + ai * pv.x;
+        rv.x -= ar * qv.x - ai * qv.y;
+        rv.y -= ar * qv.y + ai * qv.x;
+        x[i] = xv;
+        r[i] = rv;
+        float2 z = rv;
+        if (P.w) {
+            z = precond2(P, i, rv);
+            q[i] = z;
+        }
+        acc.x += (double)rv.x * z.x + (double)rv.y * z.y;
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// beta = rq' / rq; rq = rq'; termination test (relativeResidualTCB).
+__global__ __launch_bounds__(kRedThreads) void cg_beta_kernel(const double2* __restrict__ partial, int nb, CgState* st,
+                                                              double* hist, double tol) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) {
+        acc.x += partial[i].x;
+        acc.y += partial[i].y;
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        const int it = st->it;
+        if (st->done) {
+            st->beta = 0.0;
+            if (hist) hist[it + 1] = 0.0;
+        } else {
+            st->beta = st->rq > 0.0 ? s.x / st->rq : 0.0;
+            st->rq = s.x;
+            if (hist) hist[it + 1] = s.x;
+            if (s.x / st->rq0 < tol || !(s.x > 0.0)) st->done = 1;
+        }
+        st->it = it + 1;
+    }
+}
+
+// p = beta p + z (z = P^2 r, held in q, or r)
+__global__ void cg_p_kernel(float2* __restrict__ p, const float2* __restrict__ z, long long n,
+                            const CgState* __restrict__ st) {
+    const float b = (float)st->beta;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float2 pv = p[i], zv = z[i];
+        p[i] = make_float2(b * pv.x + zv.x, b * pv.y + zv.y);
+    }
+}
+
+unsigned blocks_for(long long n, int threads) {
+    return (unsigned)std::max<long long>(1, std::min<long long>((n + threads - 1) / threads, 1 << 20));
+}
+
+}  // namespace
+
+// ===========================================================================
+// operator and solver objects
+// ===========================================================================
+struct gt_sense_op {
+    gt_nufft_plan* plan = nullptr;
+    int D = 0;
+    size_t N[3] = {1, 1, 1};
+    size_t nimg = 0;  // prod N
+    size_t M = 0, F = 0, C = 0;
+    int device = 0;
+    const float2* csm = nullptr;
+    const float* dcw = nullptr;
+    size_t dcw_len = 0;
+    DevBuf coil_img;   // [N..., F, C]
+    DevBuf coil_smp;   // [M, F, C]
+    // CG workspace
+    DevBuf r, p, q, mhd;
+    DevBuf partial, state, hist;
+    bool warmed = false;
+
+    size_t image_elems() const { return nimg * F; }
+
+    void img_dims(size_t* d, int& nd, bool coils) const {
+        nd = 0;
+        for (int i = 0; i < D; i++) d[nd++] = N[i];
+        d[nd++] = F;
+        if (coils) d[nd++] = C;
+    }
+
+    void check(gt_status s) const {
+        if (s != GT_OK) throw gt_error(s, last_error());
+    }
+
+    void mult_M(const void* in, void* out, bool accumulate, hipStream_t s) {
+        size_t di[5], ds[3] = {M, F, C};
+        int ni;
+        img_dims(di, ni, true);
+        dim3 grid(blocks_for((long long)nimg, 256), (unsigned)F);
+        hipLaunchKernelGGL(csm_mult_kernel, grid, dim3(256), 0, s, (const float2*)in, coil_img.as<float2>(), csm,
+                           (long long)nimg, int(F), int(C));
+        GT_HIP(hipGetLastError());
+        void* dst = accumulate ? coil_smp.ptr : out;
+        check(gt_nufft_compute(plan, coil_img.ptr, di, ni, dst, ds, 3, dcw, dcw_len, GT_FORWARDS_C2NC, s));
+        if (accumulate) {
+            const long long n = (long long)(M * F * C);
+            hipLaunchKernelGGL(add_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, (float2*)out,
+                               coil_smp.as<float2>(), n);
+            GT_HIP(hipGetLastError());
+        }
+    }
+
+    void mult_MH(const void* in, void* out, bool accumulate, hipStream_t s) {
+        size_t di[5], ds[3] = {M, F, C};
+        int ni;
+        img_dims(di, ni, true);
+        check(gt_nufft_compute(plan, in, ds, 3, coil_img.ptr, di, ni, dcw, dcw_len, GT_BACKWARDS_NC2C, s));
+        dim3 grid(blocks_for((long long)nimg, 256), (unsigned)F);
+        hipLaunchKernelGGL(csm_conj_sum_kernel, grid, dim3(256), 0, s, coil_img.as<float2>(), (float2*)out, csm,
+                           (long long)nimg, int(F), int(C), accumulate ? 1 : 0);
+        GT_HIP(hipGetLastError());
+    }
+
+    void mult_MH_M(const void* in, void* out, bool accumulate, hipStream_t s) {
+        mult_M(in, coil_smp.ptr, false, s);
+        // coil_smp is consumed by NC2C before mult_MH writes coil_img
+        size_t di[5], ds[3] = {M, F, C};
+        int ni;
+        img_dims(di, ni, true);
+        check(gt_nufft_compute(plan, coil_smp.ptr, ds, 3, coil_img.ptr, di, ni, dcw, dcw_len, GT_BACKWARDS_NC2C, s));
+        dim3 grid(blocks_for((long long)nimg, 256), (unsigned)F);
+        hipLaunchKernelGGL(csm_conj_sum_kernel, grid, dim3(256), 0, s, coil_img.as<float2>(), (float2*)out, csm,
+                           (long long)nimg, int(F), int(C), accumulate ? 1 : 0);
+        GT_HIP(hipGetLastError());
+    }
+
+    void reserve_cg(int iterations) {
+        const size_t bytes = image_elems() * sizeof(float2);
+        r.reserve(bytes);
+        p.reserve(bytes);
+        q.reserve(bytes);
+        mhd.reserve(bytes);
+        partial.reserve(kRedBlocks * sizeof(double2));
+        state.reserve(sizeof(CgState));
+        hist.reserve(size_t(iterations + 1) * sizeof(double));
+    }
+
+    // One E^H E application grows every buffer the solve uses (the plan's
+    // grid, partial tiles, ...), so a later graph capture allocates nothing.
+    void warm(hipStream_t s) {
+        if (warmed) return;
+        mult_MH_M(p.ptr, q.ptr, false, s);
+        GT_HIP(hipStreamSynchronize(s));
+        warmed = true;
+    }
+};
+
+namespace {
+
+CgParams cg_params(const gt_sense_op* op, const gt_cg_desc* d) {
+    CgParams P{};
+    P.n = (long long)op->image_elems();
+    P.we_re = float(d->encoding_weight);
+    P.kappa = float(d->kappa);
+    P.w = static_cast<const float2*>(d->precond);
+    P.wlen = (long long)d->precond_len;
+    P.tol = d->tc_tolerance;
+    return P;
+}
+
+void validate_desc(const gt_cg_desc* d) {
+    GT_REQUIRE(d, GT_ERR_INVALID_ARGUMENT, "null CG descriptor");
+    GT_REQUIRE(d->iterations >= 0, GT_ERR_INVALID_ARGUMENT, "negative iteration count");
+    GT_REQUIRE(!d->precond || d->precond_len > 0, GT_ERR_INVALID_ARGUMENT, "preconditioner with zero length");
+}
+
+// Host callback of the plain solver: the reference reads every dot product
+// back (cgSolver.h:333-365 through cuNDArray_blas dot); here each finished
+// reduction is copied to the host and waited for.
+void host_read(const gt_sense_op* op, CgState* h, hipStream_t s) {
+    GT_HIP(hipMemcpyAsync(h, op->state.ptr, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+}
+
+// The whole solve on one stream.  host_sync: read the scalars back after each
+// reduction (plain method); otherwise no host round trip (capturable).
+void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, bool host_sync, hipStream_t s) {
+    const CgParams P = cg_params(op, d);
+    GT_REQUIRE(!P.w || (long long)op->image_elems() % P.wlen == 0, GT_ERR_SHAPE,
+               "preconditioner length does not divide the image size");
+    const unsigned nb = (unsigned)std::min<long long>(kRedBlocks, (P.n + kRedThreads - 1) / kRedThreads);
+    double2* part = op->partial.as<double2>();
+    CgState* st = op->state.as<CgState>();
+    double* hist = op->hist.as<double>();
+    float2* xf = static_cast<float2*>(x);
+    CgState h{};
+    // rhs = w_E E^H d (cgSolver::compute_rhs), x = 0, r = p = rhs (initialize)
+    op->mult_MH(data, op->mhd.ptr, false, s);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->mhd.as<float2>(), xf,
+                       op->r.as<float2>(), op->p.as<float2>(), P, part);
+    hipLaunchKernelGGL(cg_init_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st, hist);
+    GT_HIP(hipGetLastError());
+    if (host_sync) {
+        host_read(op, &h, s);
+        if (h.done) return;
+    }
+    for (int it = 0; it < d->iterations; it++) {
+        // q = (w_E E^H E + kappa I) p
+        op->mult_MH_M(op->p.ptr, op->q.ptr, false, s);
+        hipLaunchKernelGGL(cg_q_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->p.as<float2>(), op->q.as<float2>(), P,
+                           part);
+        hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st);
+        GT_HIP(hipGetLastError());
+        if (host_sync) host_read(op, &h, s);
+        hipLaunchKernelGGL(cg_xr_kernel, dim3(nb), dim3(kRedThreads), 0, s, xf, op->r.as<float2>(),
+                           op->p.as<float2>(), op->q.as<float2>(), P, st, part);
+        hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st, hist, P.tol);
+        GT_HIP(hipGetLastError());
+        if (host_sync) {
+            host_read(op, &h, s);
+            if (h.done) return;  // relativeResidualTCB: terminate
+        }
+        const float2* z = P.w ? op->q.as<float2>() : op->r.as<float2>();
+        hipLaunchKernelGGL(cg_p_kernel, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, op->p.as<float2>(), z, P.n, st);
+        GT_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+struct gt_cg_batch {
+    std::vector<gt_sense_op*> ops;
+    std::vector<const void*> data;
+    std::vector<void*> x;
+    gt_cg_desc desc{};
+    int mode = GT_CG_PLAIN;
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;
+    ~gt_cg_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : events) (void)hipEventDestroy(e);
+        for (auto s : streams) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+constexpr int kBranches = 8;  // parallel branches of the batched graph
+
+// Problems longest first onto the least loaded branch (work ~ samples x coils
+// + image x coils).
+std::vector<std::vector<int>> assign_branches(const std::vector<gt_sense_op*>& ops, int nbr) {
+    std::vector<int> order(ops.size());
+    std::iota(order.begin(), order.end(), 0);
+    auto work = [&](int i) {
+        const gt_sense_op* o = ops[size_t(i)];
+        return double(o->M * o->F * o->C) + 2.25 * double(o->nimg * o->F * o->C);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::vector<std::vector<int>> br(static_cast<size_t>(nbr));
+    std::vector<double> load(static_cast<size_t>(nbr), 0.0);
+    for (int i : order) {
+        const size_t k = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+        br[k].push_back(i);
+        load[k] += work(i);
+    }
+    return br;
+}
+
+// Captures on a stream of its own (the caller's may be the null stream, which
+// cannot be captured); the instantiated graph is launched on the caller's.
+void build_graph(gt_cg_batch* b) {
+    const int nbr = std::min<int>(kBranches, int(b->ops.size()));
+    auto br = assign_branches(b->ops, nbr);
+    for (int i = 0; i < nbr + 1; i++) {
+        hipStream_t st;
+        GT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->streams.push_back(st);
+    }
+    hipStream_t s = b->streams[size_t(nbr)];
+    for (int i = 0; i < nbr + 1; i++) {
+        hipEvent_t e;
+        GT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->events.push_back(e);
+    }
+    // fork from s to the branch streams, each branch runs its problems' whole
+    // solves back to back, join back into s
+    GT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    GT_HIP(hipEventRecord(b->events[0], s));
+    for (int k = 0; k < nbr; k++) {
+        hipStream_t bs = b->streams[size_t(k)];
+        GT_HIP(hipStreamWaitEvent(bs, b->events[0], 0));
+        for (int i : br[size_t(k)]) cg_solve(b->ops[size_t(i)], b->data[size_t(i)], b->x[size_t(i)], &b->desc, false, bs);
+        GT_HIP(hipEventRecord(b->events[size_t(k) + 1], bs));
+        GT_HIP(hipStreamWaitEvent(s, b->events[size_t(k) + 1], 0));
+    }
+    GT_HIP(hipStreamEndCapture(s, &b->graph));
+    GT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+}  // namespace
+
+extern "C" {
+
+gt_status gt_sense_create(gt_sense_op** out, gt_nufft_plan* plan, const void* csm, size_t ncoils, const float* dcw,
+                          size_t dcw_len) {
+    return guarded([&] {
+        GT_REQUIRE(out && plan && csm, GT_ERR_INVALID_ARGUMENT, "null operator, plan or coil maps");
+        *out = nullptr;
+        GT_REQUIRE(ncoils > 0, GT_ERR_INVALID_ARGUMENT, "no coils");
+        GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
+        auto op = std::make_unique<gt_sense_op>();
+        op->plan = plan;
+        size_t N[4] = {0, 0, 0, 0};
+        auto chk = [](gt_status s) {
+            if (s != GT_OK) throw gt_error(s, last_error());
+        };
+        chk(gt_nufft_get_matrix_size(plan, N));
+        chk(gt_nufft_get_num_samples_frames(plan, &op->M, &op->F));
+        GT_REQUIRE(op->M > 0 && op->F > 0, GT_ERR_NOT_PREPROCESSED, "plan has not been preprocessed");
+        op->D = 0;
+        op->nimg = 1;
+        for (int d = 0; d < 3 && N[d] > 0; d++) {
+            op->N[d] = N[d];
+            op->nimg *= N[d];
+            op->D = d + 1;
+        }
+        GT_REQUIRE(op->D >= 2, GT_ERR_UNSUPPORTED, "the SENSE operator needs a 2D or 3D plan");
+        op->C = ncoils;
+        op->csm = static_cast<const float2*>(csm);
+        op->dcw = dcw;
+        op->dcw_len = dcw_len;
+        GT_REQUIRE(op->nimg * op->F * op->C < (size_t(1) << 40), GT_ERR_UNSUPPORTED, "problem too large");
+        GT_HIP(hipGetDevice(&op->device));
+        op->coil_img.reserve(op->nimg * op->F * op->C * sizeof(float2));
+        op->coil_smp.reserve(op->M * op->F * op->C * sizeof(float2));
+        *out = op.release();
+    });
+}
+
+gt_status gt_sense_destroy(gt_sense_op* op) {
+    return guarded([&] { delete op; });
+}
+
+gt_status gt_sense_mult_M(gt_sense_op* op, const void* in, void* out, int accumulate, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(op && in && out, GT_ERR_INVALID_ARGUMENT, "null argument");
+        op->mult_M(in, out, accumulate != 0, S(stream));
+    });
+}
+
+gt_status gt_sense_mult_MH(gt_sense_op* op, const void* in, void* out, int accumulate, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(op && in && out, GT_ERR_INVALID_ARGUMENT, "null argument");
+        op->mult_MH(in, out, accumulate != 0, S(stream));
+    });
+}
+
+gt_status gt_sense_mult_MH_M(gt_sense_op* op, const void* in, void* out, int accumulate, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(op && in && out, GT_ERR_INVALID_ARGUMENT, "null argument");
+        op->mult_MH_M(in, out, accumulate != 0, S(stream));
+    });
+}
+
+gt_status gt_cgsense_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* desc, double* rq_history,
+                           gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(op && data && x, GT_ERR_INVALID_ARGUMENT, "null argument");
+        validate_desc(desc);
+        hipStream_t s = S(stream);
+        op->reserve_cg(desc->iterations);
+        cg_solve(op, data, x, desc, false, s);
+        if (rq_history) {
+            GT_HIP(hipMemcpyAsync(rq_history, op->hist.ptr, size_t(desc->iterations + 1) * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+            GT_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
+gt_status gt_cgsense_batch_create(gt_cg_batch** out, gt_sense_op* const* ops, const void* const* data,
+                                  void* const* x, int nprob, const gt_cg_desc* desc, int mode, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(out && ops && data && x && nprob > 0, GT_ERR_INVALID_ARGUMENT, "null or empty problem set");
+        *out = nullptr;
+        validate_desc(desc);
+        GT_REQUIRE(mode == GT_CG_PLAIN || mode == GT_CG_BATCHED, GT_ERR_INVALID_ARGUMENT, "unknown batch mode");
+        auto b = std::make_unique<gt_cg_batch>();
+        b->desc = *desc;
+        b->mode = mode;
+        GT_HIP(hipGetDevice(&b->device));
+        hipStream_t s = S(stream);
+        for (int i = 0; i < nprob; i++) {
+            GT_REQUIRE(ops[i] && data[i] && x[i], GT_ERR_INVALID_ARGUMENT, "null problem entry");
+            for (int j = 0; j < i; j++)
+                GT_REQUIRE(ops[j] != ops[i] && ops[j]->plan != ops[i]->plan, GT_ERR_INVALID_ARGUMENT,
+                           "problems of one batch need their own operator and plan (the branches run concurrently)");
+            ops[i]->reserve_cg(desc->iterations);
+            ops[i]->warm(s);
+            b->ops.push_back(ops[i]);
+            b->data.push_back(data[i]);
+            b->x.push_back(x[i]);
+        }
+        if (mode == GT_CG_BATCHED) build_graph(b.get());
+        *out = b.release();
+    });
+}
+
+gt_status gt_cgsense_batch_run(gt_cg_batch* b, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(b, GT_ERR_INVALID_ARGUMENT, "null batch");
+        hipStream_t s = S(stream);
+        if (b->mode == GT_CG_BATCHED) {
+            GT_HIP(hipGraphLaunch(b->exec, s));
+        } else {
+            for (size_t i = 0; i < b->ops.size(); i++) cg_solve(b->ops[i], b->data[i], b->x[i], &b->desc, true, s);
+        }
+    });
+}
+
+gt_status gt_cgsense_batch_destroy(gt_cg_batch* b) {
+    return guarded([&] { delete b; });
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// gt_common.h -- host-side plumbing shared by the translation units of
+// libgtnufft: status exceptions, the guarded C-ABI wrapper, grow-only device
+// buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/gt_nufft.h"
+
+namespace gt_host {
+
+// Thread-local message behind gt_last_error() (defined once, in gt_plan.hip).
+std::string& last_error();
+
+struct gt_error : std::runtime_error {
+    gt_status status;
+    gt_error(gt_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define GT_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            std::ostringstream os_;                                                       \
+            os_ << #call << ": " << hipGetErrorString(e_);                                \
+            throw ::gt_host::gt_error(e_ == hipErrorOutOfMemory ? GT_ERR_OUT_OF_MEMORY : GT_ERR_DEVICE, \
+                                      os_.str());                                         \
+        }                                                                                 \
+    } while (0)
+
+#define GT_REQUIRE(cond, status, msg)                             \
+    do {                                                          \
+        if (!(cond)) throw ::gt_host::gt_error((status), (msg)); \
+    } while (0)
+
+// Runs f, turning exceptions into a status and the thread-local message.
+template <class F>
+gt_status guarded(F&& f) {
+    try {
+        f();
+        return GT_OK;
+    } catch (const gt_error& e) {
+        last_error() = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        last_error() = "host allocation failed";
+        return GT_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        last_error() = e.what();
+        return GT_ERR_INTERNAL;
+    }
+}
+
+inline hipStream_t S(gt_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        if (ptr) GT_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        GT_HIP(hipMalloc(&ptr, n));
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace gt_host

@@ -35,76 +35,21 @@
 #include <vector>
 
 #include "../../include/gt_nufft.h"
+#include "gt_common.h"
 #include "gt_kb.h"
 #include "gt_kernels.h"
 #include "gt_kernels_fast.h"
 #include "gt_kernels_reg.h"
 #include "gt_fft.h"
 
-namespace {
-
-thread_local std::string g_last_error;
-
-struct gt_error : std::runtime_error {
-    gt_status status;
-    gt_error(gt_status s, const std::string& m) : std::runtime_error(m), status(s) {}
-};
-
-#define GT_HIP(call)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess) {                                                           \
-            std::ostringstream os_;                                                       \
-            os_ << #call << ": " << hipGetErrorString(e_);                                \
-            throw gt_error(e_ == hipErrorOutOfMemory ? GT_ERR_OUT_OF_MEMORY : GT_ERR_DEVICE, \
-                           os_.str());                                                    \
-        }                                                                                 \
-    } while (0)
-
-#define GT_REQUIRE(cond, status, msg)                \
-    do {                                             \
-        if (!(cond)) throw gt_error((status), (msg)); \
-    } while (0)
-
-template <class F>
-gt_status guarded(F&& f) {
-    try {
-        f();
-        return GT_OK;
-    } catch (const gt_error& e) {
-        g_last_error = e.what();
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return GT_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return GT_ERR_INTERNAL;
-    }
+std::string& gt_host::last_error() {
+    thread_local std::string message;
+    return message;
 }
 
-inline hipStream_t S(gt_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+namespace {
 
-// Grow-only device buffer.
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    void reserve(size_t n) {
-        if (n <= bytes) return;
-        if (ptr) GT_HIP(hipFree(ptr));
-        ptr = nullptr;
-        bytes = 0;
-        GT_HIP(hipMalloc(&ptr, n));
-        bytes = n;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const { return static_cast<T*>(ptr); }
-};
+using namespace gt_host;
 
 std::once_flag g_rocfft_once;
 
@@ -1154,7 +1099,7 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
 // ===========================================================================
 extern "C" {
 
-const char* gt_last_error(void) { return g_last_error.c_str(); }
+const char* gt_last_error(void) { return gt_host::last_error().c_str(); }
 
 gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
     return guarded([&] {

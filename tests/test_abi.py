@@ -15,8 +15,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "gt_nufft.h")).read()
-    return re.findall(r"^\s*(?:gt_status|const char\*)\s+(gt_\w+)\s*\(", text, flags=re.M)
+    out = []
+    for h in ("gt_nufft.h", "gt_cgsense.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        out += re.findall(r"^\s*(?:gt_status|const char\*)\s+(gt_\w+)\s*\(", text, flags=re.M)
+    return out
 
 
 def test_header_and_binding_agree():
