@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
+    ap.add_argument("--max-items", type=int, default=0,
+                    help="entries per gridding work item (gt_nufft_set_tiling); 0 keeps the default")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the NUFFT shards by coil "
@@ -152,6 +154,8 @@ def main():
     samples = tj.complex_normal(rng, (C, 1, M))
 
     plan = ga.NFFTPlan(N, None, W=5.5, os_factor=1.5)
+    if args.max_items:
+        plan.set_tiling(None, args.max_items)
     d_traj = DeviceArray.from_numpy(traj)
     t0 = time.perf_counter()
     plan.preprocess(d_traj)

@@ -47,8 +47,10 @@ struct WorkItem {
     uint32_t key;    // frame * ntiles_total + tile
     uint32_t begin;  // range into the tile-sorted sample list
     uint32_t end;
-    int32_t partial; // -1: write the grid; else the partial-tile slot
+    int32_t partial; // -1: write the grid; kNoopItem: do nothing; else the partial-tile slot
 };
+// Placeholder item (padding of the per-XCD interleave, empty problems).
+constexpr int32_t kNoopItem = -2;
 
 struct RedItem {
     uint32_t key;
@@ -263,6 +265,7 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
     const float* __restrict__ dcw, long long dcw_len, int accumulate) {
     extern __shared__ float lds[];
     const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
     const int c0 = blockIdx.y * CC;
     const int ncc = min(CC, C - c0);
     const int TC = g.tile_cells;

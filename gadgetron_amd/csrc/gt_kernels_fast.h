@@ -122,7 +122,8 @@ __global__ void c2nc_table2d_kernel(const float* __restrict__ p, const uint32_t*
 // NC2C table, per (8 x 8 tile, sample) pair e, in two arrays:
 //   wt[16 e + c]     weight of tile column c (c = 0..7), then of tile row c - 8
 //                    (zero where the window does not reach)
-//   hd[2 e], hd[2 e + 1]   sample index j, mask of the tile rows the window covers
+//   hd[2 e]          sample index j
+//   hd[2 e + 1]      bits 0-7: tile rows the window covers, bits 8-15: tile columns
 // Cells are matched by their wrapped coordinate, so windows that wrap around the
 // grid edge need no special case.
 constexpr int kTabNW = 16;  // weight floats per NC2C entry (one 64-byte line)
@@ -138,7 +139,7 @@ __global__ void nc2c_table2d_kernel(const float* __restrict__ p, const uint32_t*
     const int tile = (int)(keys[e] % (uint32_t)g.ntiles_total);
     const int org[2] = {(tile % g.ntiles[0]) * g.tile[0], (tile / g.ntiles[0]) * g.tile[1]};
     float tw[2][kTabTile];
-    uint32_t rowmask = 0;
+    uint32_t mask[2] = {0u, 0u};  // tile columns / rows the window covers
     for (int d = 0; d < 2; d++) {
         float w[KW];
         int lo;
@@ -150,10 +151,11 @@ __global__ void nc2c_table2d_kernel(const float* __restrict__ p, const uint32_t*
             const int l = wrap_cell(lo + i, g.nos[d]) - org[d];
             if (l >= 0 && l < kTabTile) {
                 tw[d][l] = w[i];
-                if (d == 1) rowmask |= 1u << l;
+                mask[d] |= 1u << l;
             }
         }
     }
+    const uint32_t rowmask = mask[1] | (mask[0] << 8);
     float4* t = reinterpret_cast<float4*>(wt + e * kTabNW);
     t[0] = make_float4(tw[0][0], tw[0][1], tw[0][2], tw[0][3]);
     t[1] = make_float4(tw[0][4], tw[0][5], tw[0][6], tw[0][7]);

@@ -30,6 +30,12 @@ constexpr int kWaves = 4;      // wavefronts per C2NC workgroup
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 constexpr int kNK = 16;  // NC2C table entries per chunk moved through LDS
+constexpr int kLoadCoils = kCoilsPerWave / (kLanes / kNK);  // coils per lane in a chunk's sample loads
+constexpr int kXsRow = 2 * kCoilsPerWave + 2;  // floats per entry of the sample transpose (+2: banks)
+// LDS floats of nc2c_reg2d_kernel: max(write-out buffer, loop buffers)
+constexpr int kNC2CLoopFloats = 2 * kNK * 4 * 4 + 2 * 2 * kNK + kNK * kXsRow;
+constexpr int kNC2CLdsFloats = ((kRT * kRT / 2 * kSubP > kNC2CLoopFloats ? kRT * kRT / 2 * kSubP : kNC2CLoopFloats) + 3) & ~3;
+static_assert(kNK == 16 && kLanes / kNK * kLoadCoils == kCoilsPerWave, "sample-load lane mapping");
 
 // Loads chunk entries [cb, cb + kNK) of an NC2C table (entries [begin, end),
 // end > begin) into registers, one float4 of weights per lane and one head
@@ -51,23 +57,29 @@ __device__ __forceinline__ void nc2c_chunk_load(const float* __restrict__ wt, co
 // DM: how dcw is applied (dcw_mode): 0 none, 1 per sample (scalar loads),
 // 2 per coil element (loaded with the samples).
 template <int DM>
-__global__ __launch_bounds__(64) void nc2c_reg2d_kernel(
+__global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
     const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
     const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
     int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
     int accumulate) {
-    __shared__ float xp[kRT * kRT / 2 * kSubP];  // half-tile transpose buffer
-    __shared__ float4 sw[2][kNK * 4];            // weights of two table chunks
-    __shared__ uint32_t sh[2][2 * kNK];          // heads (j, row mask) of two table chunks
+    // LDS: the gridding loop's buffers and the write-out's transpose buffer are
+    // never live together (the write-out starts with a barrier)
+    __shared__ float4 smem[kNC2CLdsFloats / 4];
+    float* xp = reinterpret_cast<float*>(smem);  // half-tile transpose buffer
+    float4 (*sw)[kNK * 4] = reinterpret_cast<float4 (*)[kNK * 4]>(smem);  // weights of two table chunks
+    uint32_t (*sh)[2 * kNK] = reinterpret_cast<uint32_t (*)[2 * kNK]>(smem + 2 * kNK * 4);  // heads (j, rows)
+    float* xs = reinterpret_cast<float*>(smem + 2 * kNK * 4 + kNK);  // a chunk's samples [entry][coil][re, im]
     const int lane = threadIdx.x;
     const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
     const int c0 = blockIdx.y * kCoilsPerWave;
-    const int coil = c0 + (lane >> 1);
-    const bool on = coil < C;
     const long long FM = (long long)F * M;
-    const int lc = on ? coil : 0;  // lanes past the last coil read coil 0 and store nothing
-    const float* sl = samples + 2 * (long long)lc * FM + (lane & 1);
     const uint32_t nch = (it.end - it.begin + kNK - 1) / kNK;
+    // sample loads: lane = (entry kk, coil quarter cq), 8 coils each, so one
+    // load instruction reads 4 coils x 16 entries -- runs of consecutive
+    // samples of a spoke are contiguous per coil
+    const int kk = lane & (kNK - 1), cq = lane >> 4;
+    const float2* s2 = reinterpret_cast<const float2*>(samples);
 
     float acc[kRT * kRT];
 #pragma unroll
@@ -75,35 +87,46 @@ __global__ __launch_bounds__(64) void nc2c_reg2d_kernel(
 
     // Pipeline: while chunk c is gridded from LDS buffer c & 1, the samples of
     // chunk c + 1 are in flight and the table of chunk c + 2 is in registers.
-    constexpr int kDV = DM == 2 ? kNK : 1;
+    constexpr int kDV = DM == 2 ? kLoadCoils : 1;
     float4 wreg;
     uint32_t hreg;
-    float svA[kNK], svB[kNK], dvA[kDV], dvB[kDV];
+    float2 svA[kLoadCoils], svB[kLoadCoils];
+    float dvA[kDV], dvB[kDV];
 #pragma unroll
-    for (int k = 0; k < kNK; k++) svA[k] = svB[k] = 0.0f;
+    for (int u = 0; u < kLoadCoils; u++) svA[u] = svB[u] = make_float2(0.0f, 0.0f);
 #pragma unroll
-    for (int k = 0; k < kDV; k++) dvA[k] = dvB[k] = 1.0f;
+    for (int u = 0; u < kDV; u++) dvA[u] = dvB[u] = 1.0f;
 
     auto stage = [&](int b) {
         sw[b][lane] = wreg;
         if (lane < 2 * kNK) sh[b][lane] = hreg;
     };
-    auto issue = [&](int b, float (&sv)[kNK], float (&dv)[kDV]) {
-        const int hv = (int)sh[b][lane & 31];
+    auto issue = [&](int b, float2 (&sv)[kLoadCoils], float (&dv)[kDV]) {
+        const uint32_t j = sh[b][2 * kk];
 #pragma unroll
-        for (int k = 0; k < kNK; k++) {
-            const uint32_t j = (uint32_t)__builtin_amdgcn_readlane(hv, 2 * k);
-            sv[k] = sl[2 * (long long)j];
-            if constexpr (DM == 2) dv[k] = dcw[dcw_index(lc, j, FM, dcw_len)];
+        for (int u = 0; u < kLoadCoils; u++) {
+            int c = c0 + cq * kLoadCoils + u;
+            c = c < C ? c : c0;  // coils past the last one repeat a valid coil, never used
+            sv[u] = s2[(long long)c * FM + j];
+            if constexpr (DM == 2) dv[u] = dcw[dcw_index(c, j, FM, dcw_len)];
         }
     };
-    auto grid_chunk = [&](int b, const float (&sv)[kNK], const float (&dv)[kDV]) {
+    auto grid_chunk = [&](int b, const float2 (&sv)[kLoadCoils], const float (&dv)[kDV]) {
+        // redistribute the chunk's samples: lane = (coil, re/im) reads entry k
+#pragma unroll
+        for (int u = 0; u < kLoadCoils; u++) {
+            float2 v = sv[u];
+            if constexpr (DM == 2) {
+                v.x *= dv[u];
+                v.y *= dv[u];
+            }
+            *reinterpret_cast<float2*>(&xs[kk * kXsRow + 2 * (cq * kLoadCoils + u)]) = v;
+        }
         const int hv = (int)sh[b][lane & 31];
         float dk[kNK];
 #pragma unroll
         for (int k = 0; k < kNK; k++) {
             if constexpr (DM == 1) dk[k] = dcw[(uint32_t)__builtin_amdgcn_readlane(hv, 2 * k)];
-            else if constexpr (DM == 2) dk[k] = dv[k];
             else dk[k] = 1.0f;
         }
 #pragma unroll
@@ -112,7 +135,11 @@ __global__ __launch_bounds__(64) void nc2c_reg2d_kernel(
             const float4 x0 = sw[b][4 * k], x1 = sw[b][4 * k + 1], y0 = sw[b][4 * k + 2], y1 = sw[b][4 * k + 3];
             const float wc[kRT] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
             const float wr[kRT] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
-            const float s = DM ? sv[k] * dk[k] : sv[k];
+            const float sk = xs[k * kXsRow + lane];
+            const float s = DM == 1 ? sk * dk[k] : sk;
+            // rows: bits 0-7 covered tile rows (a scalar test per row); all 8
+            // columns are updated (zero weight outside the window): skipping
+            // half-rows costs more in branches than the FMAs it saves
 #pragma unroll
             for (int cy = 0; cy < kRT; cy++) {
                 if (!((rows >> cy) & 1u)) continue;
@@ -214,9 +241,10 @@ __global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restr
 // MODE 0: no dcw, no accumulation; 1: one dcw weight per sample, no
 // accumulation; 2: anything else (per-element dcw, accumulation).
 // LDS holds at most 3 workgroups per CU (3 waves per SIMD): the waves-per-EU
-// range lets the scheduler keep a whole window's reads in flight.
+// range lets the scheduler keep a whole window's reads, and the staging's
+// loads, in flight.
 template <int KW, int MODE>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 4))) void c2nc_multi2d_kernel(
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 3))) void c2nc_multi2d_kernel(
     const float* __restrict__ grid, float* __restrict__ out, const uint32_t* __restrict__ tab,
     const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g, Layout L, int sign,
     const float* __restrict__ dcw, long long dcw_len, int accumulate) {
@@ -225,6 +253,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = wave_uniform(tid >> 6);
     const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
     const int c0 = blockIdx.y * kCoilsPerWave;
     const int fr = (int)floorf(g.radius);
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
@@ -233,39 +262,51 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     const uint32_t ne = it.end - it.begin;  // at most kC2NCSeg
     uint4* st = reinterpret_cast<uint4*>(sg + c2nc_lds_floats(kRT, kRT, KW));
 
-    // the item's table entries, 16 dwords each
-    for (uint32_t i = tid; i < ne * 4; i += 64 * kWaves)
-        st[i] = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + i];
+    // the item's table entries, 16 dwords each (one uint4 per thread), loaded
+    // together with the grid below and stored after it
+    static_assert(kC2NCSeg * 4 <= 64 * kWaves, "one table uint4 per thread");
+    const bool tlive = (uint32_t)tid < ne * 4;
+    uint4 tv = make_uint4(0u, 0u, 0u, 0u);
+    if (tlive) tv = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + tid];
 
     // tile + halo: a thread owns one float (cell ex, re or im) of a staged row
     // and walks the E1 rows for 4 coils; the row wrap and centring sign are
-    // worked out once per row
+    // worked out once per row.  All E1 x 4 loads are issued before the first
+    // LDS store, so the staging pays the memory latency once.
     {
+        constexpr int kU = kCoilsPerWave / 8;
         const int q = tid & 31, cg = tid >> 5;  // 8 coil groups of 32 threads
-        if (q < 2 * E0) {
-            const int ex = q >> 1, comp = q & 1;
-            int gx = sx + ex;
-            gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
-            const long long cstride = 2 * L.sb * (long long)F;  // next coil
-            const float* gcol = grid + 2 * (long long)gx + comp + 2 * L.sb * (long long)f;
+        const bool live = q < 2 * E0;
+        const int ex = live ? q >> 1 : 0, comp = q & 1;
+        int gx = sx + ex;
+        gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
+        const long long cstride = 2 * L.sb * (long long)F;  // next coil
+        const float* gcol = grid + 2 * (long long)gx + comp + 2 * L.sb * (long long)f;
+        float v[E1][kU];
+#pragma unroll
+        for (int ey = 0; ey < E1; ey++) {
+            int gy = sy0 + ey;
+            gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
+            const float* prow = gcol + 2 * L.sy * (long long)gy;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int cl = cg + 8 * u;
+                v[ey][u] = (live && c0 + cl < C) ? prow[(long long)(c0 + cl) * cstride] : 0.0f;
+            }
+        }
+        if (live) {
             float* dcol = sg + ex * kSubP + comp;
 #pragma unroll
             for (int ey = 0; ey < E1; ey++) {
-                int gy = sy0 + ey;
-                gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
+                const int gy = sy0 + ey;  // parity of the wrapped row equals that of gy (even N_os)
                 const float sgn = (sign && ((gx + gy) & 1)) ? -1.0f : 1.0f;
-                const float* prow = gcol + 2 * L.sy * (long long)gy;
 #pragma unroll
-                for (int u = 0; u < kCoilsPerWave / 8; u++) {
-                    const int cl = cg + 8 * u;
-                    float v = 0.0f;
-                    if (c0 + cl < C) v = prow[(long long)(c0 + cl) * cstride] * sgn;
-                    dcol[ey * E0 * kSubP + 2 * cl] = v;
-                }
+                for (int u = 0; u < kU; u++) dcol[ey * E0 * kSubP + 2 * (cg + 8 * u)] = v[ey][u] * sgn;
             }
         }
         for (int i = tid; i < (E0 + 1) * kSubP; i += 64 * kWaves) sg[E0 * E1 * kSubP + i] = 0.0f;
     }
+    if (tlive) st[tid] = tv;
     __syncthreads();
 
     // lanes past the last coil mirror coil c0 (its staged values, its output

@@ -674,6 +674,58 @@ std::vector<uint32_t> bin_home(gt_nufft_plan* p, const gt::GridGeom& g, Binning&
     return cnt;
 }
 
+// Workgroup order.  MI355X hands workgroup i of a launch to XCD i mod 8, and
+// each XCD has its own L2.  Items (created in key order: frame, tile row, tile
+// column) are cut into kXcds contiguous runs of equal work -- horizontal bands
+// of tiles, so neighbouring tiles, which share halo cells and samples, meet in
+// one L2 -- each run is sorted longest first (the dispatcher hands out
+// workgroups in index order), and the runs are interleaved, padded with no-op
+// items so that item i always runs on XCD i mod kXcds.  GT_NUFFT_XCD=0 keeps
+// a single longest-first order instead.
+constexpr int kXcds = 8;
+
+bool xcd_ordering() {
+    static const bool on = [] {
+        const char* e = std::getenv("GT_NUFFT_XCD");
+        return !(e && std::string(e) == "0");
+    }();
+    return on;
+}
+
+std::vector<gt::WorkItem> xcd_order(std::vector<gt::WorkItem> items) {
+    auto longest_first = [](std::vector<gt::WorkItem>& v) {
+        std::stable_sort(v.begin(), v.end(), [](const gt::WorkItem& a, const gt::WorkItem& b) {
+            return (a.end - a.begin) > (b.end - b.begin);
+        });
+    };
+    if (!xcd_ordering() || items.size() < size_t(4 * kXcds)) {
+        longest_first(items);
+        return items;
+    }
+    // a fixed cost per item (staging, write-out) plus one unit per entry
+    auto work = [](const gt::WorkItem& w) { return double(w.end - w.begin) + 32.0; };
+    double total = 0.0;
+    for (const auto& w : items) total += work(w);
+    std::vector<std::vector<gt::WorkItem>> run(kXcds);
+    double cum = 0.0;
+    for (const auto& w : items) {
+        const int g = std::min(kXcds - 1, int(cum / total * kXcds));
+        run[size_t(g)].push_back(w);
+        cum += work(w);
+    }
+    size_t longest = 0;
+    for (auto& r : run) {
+        longest_first(r);
+        longest = std::max(longest, r.size());
+    }
+    std::vector<gt::WorkItem> out;
+    out.reserve(longest * kXcds);
+    for (size_t k = 0; k < longest; k++)
+        for (int g = 0; g < kXcds; g++)
+            out.push_back(k < run[size_t(g)].size() ? run[size_t(g)][k] : gt::WorkItem{0, 0, 0, gt::kNoopItem});
+    return out;
+}
+
 // Work items over per-key counts: one per key, keys above L entries split into
 // segments (with partial tiles summed in segment order when allowed).
 void make_items(gt_nufft_plan* p, Binning& b, const std::vector<uint32_t>& counts, size_t L, bool partials,
@@ -702,11 +754,8 @@ void make_items(gt_nufft_plan* p, Binning& b, const std::vector<uint32_t>& count
         }
         begin += n;
     }
-    // Longest items first: the dispatcher hands out workgroups in index order.
-    std::stable_sort(items.begin(), items.end(), [](const gt::WorkItem& a, const gt::WorkItem& b2) {
-        return (a.end - a.begin) > (b2.end - b2.begin);
-    });
-    if (items.empty()) items.push_back({0, 0, 0, -2});  // keep one launchable (no-op) item
+    items = xcd_order(std::move(items));
+    if (items.empty()) items.push_back({0, 0, 0, gt::kNoopItem});  // keep one launchable item
     b.n_items = int(items.size());
     b.n_red = int(red.size());
     b.n_partials = n_partials;
