@@ -111,6 +111,114 @@ def copy_probe(mib, reps):
           flush=True)
 
 
+def ndft_error(img, out, wl, rng, nsamp=512):
+    """NUFFT approximation error of the run's last FORWARDS_C2NC against an FP64
+    direct NDFT: coil 0, nsamp random samples, y = dcw * NDFT(x) / sqrt(G)."""
+    from oracle import pyoracle as po
+    N, traj, dcf = wl["N"], wl["traj"], wl["dcf"]
+    G = float(np.prod([int(n * 1.5) for n in N]))
+    idx = np.sort(rng.choice(traj.shape[1], size=min(nsamp, traj.shape[1]), replace=False))
+    k = traj[0, idx].astype(np.float64)
+    exact = po.ndft(k, N, x=img[0, 0].astype(np.complex128))[0] / np.sqrt(G) * dcf[0, idx]
+    got = out[0, 0, idx].astype(np.complex128)
+    return {"rel_l2": float(np.linalg.norm(got - exact) / np.linalg.norm(exact)), "samples": int(len(idx)),
+            "coil": 0, "reference": "FP64 direct NDFT / sqrt(G), times dcw"}
+
+
+def ellipse_phantom(rng, N, n=10):
+    """Sum of n random ellipses (Shepp-Logan-like) on an N x N grid."""
+    y, x = np.mgrid[-1:1:N * 1j, -1:1:N * 1j]
+    img = np.zeros((N, N), np.float64)
+    for _ in range(n):
+        cx, cy = rng.uniform(-0.6, 0.6, 2)
+        a, b = rng.uniform(0.05, 0.45, 2)
+        th = rng.uniform(0, np.pi)
+        xr = (x - cx) * np.cos(th) + (y - cy) * np.sin(th)
+        yr = -(x - cx) * np.sin(th) + (y - cy) * np.cos(th)
+        img[(xr / a) ** 2 + (yr / b) ** 2 <= 1.0] += rng.uniform(0.1, 1.0)
+    return img
+
+
+def gaussian_csm(rng, C, N):
+    """Normalised complex Gaussian bumps, sigma N/3, at random centres."""
+    y, x = np.mgrid[0:N, 0:N]
+    maps = []
+    for _ in range(C):
+        cy, cx = rng.uniform(0, N, 2)
+        maps.append(np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * (N / 3) ** 2)) * np.exp(1j * rng.uniform(0, 2 * np.pi)))
+    maps = np.array(maps)
+    maps /= np.sqrt(np.sum(np.abs(maps) ** 2, axis=0, keepdims=True))
+    return maps.astype(np.complex64)
+
+
+def batched_cgsense(nprob=64, iterations=10, reps=3, seed=3):
+    """Batched uneven CG-SENSE: 64 independent 2D radial reconstructions,
+    N in {96,128,160,192,256}, spokes U[32,512], 2N samples per spoke, coils
+    U[4,32], Gaussian-bump coil maps, 10-ellipse phantom, data = A x + CN(0,1e-3),
+    10 CG iterations; the plain per-problem solver (dot products read back to
+    the host) against the hipGraph-batched solver."""
+    import gadgetron_amd as ga
+    from gadgetron_amd import DeviceArray
+    from gadgetron_amd import trajectories as tj
+    rng = np.random.default_rng(seed)
+    keep, ops, data, xp, xb = [], [], [], [], []
+    work = 0
+    for _ in range(nprob):
+        N = int(rng.choice([96, 128, 160, 192, 256]))
+        spokes = int(rng.integers(32, 513))
+        C = int(rng.integers(4, 33))
+        traj = tj.golden_radial_2d(2 * N, spokes)
+        M = traj.shape[1]
+        plan = ga.NFFTPlan([N, N], None, W=5.5, os_factor=1.5)
+        d_traj = DeviceArray.from_numpy(traj)
+        plan.preprocess(d_traj)
+        d_csm = DeviceArray.from_numpy(gaussian_csm(rng, C, N))
+        d_dcw = DeviceArray.from_numpy(np.sqrt(tj.ramp_dcf(traj)).astype(np.float32))
+        op = ga.SenseOperator(plan, d_csm, d_dcw)
+        d_x = DeviceArray.from_numpy(ellipse_phantom(rng, N).astype(np.complex64)[None])
+        d_y = DeviceArray.zeros((C, 1, M), np.complex64)
+        op.mult_M(d_x, d_y)
+        plan.synchronize()
+        y = d_y.numpy()
+        y = (y + 1e-3 * (rng.standard_normal(y.shape) + 1j * rng.standard_normal(y.shape)) / np.sqrt(2)).astype(np.complex64)
+        d_data = DeviceArray.from_numpy(y)
+        keep += [plan, d_traj, d_csm, d_dcw, d_x, d_y]
+        ops.append(op)
+        data.append(d_data)
+        xp.append(DeviceArray.zeros((1, N, N), np.complex64))
+        xb.append(DeviceArray.zeros((1, N, N), np.complex64))
+        work += M * C
+    plain = ga.CgBatch(ops, data, xp, iterations=iterations, tc_tolerance=0.0, mode="plain")
+    batched = ga.CgBatch(ops, data, xb, iterations=iterations, tc_tolerance=0.0, mode="batched")
+    sync = ops[0].plan.synchronize
+
+    def timed(b):
+        b.run()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run()
+        sync()
+        return (time.perf_counter() - t0) / reps
+
+    tp = timed(plain)
+    tb = timed(batched)
+    num = sum(float(np.linalg.norm((a.numpy() - b.numpy()).ravel())) ** 2 for a, b in zip(xp, xb))
+    den = sum(float(np.linalg.norm(a.numpy().ravel())) ** 2 for a in xp)
+    rel = (num / den) ** 0.5
+    # one E^H d, then per iteration one E and one E^H application
+    units = work * (2 * iterations + 1) / 1e6
+    return {
+        "problems": nprob, "iterations": iterations,
+        "plain_ms": round(tp * 1e3, 3), "batched_ms": round(tb * 1e3, 3),
+        "speedup": round(tp / tb, 3),
+        "graph_branches": batched.branches,
+        "batched_Msamples_coil_per_s": round(units / tb, 1),
+        "plain_Msamples_coil_per_s": round(units / tp, 1),
+        "rel_err_batched_vs_plain": rel,
+    }
+
+
 def load_traffic(config):
     path = os.path.join(ROOT, "profiles", f"traffic_{config}.json")
     if not os.path.exists(path):
@@ -127,6 +235,7 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cg", action="store_true", help="skip the batched CG-SENSE comparison")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
     ap.add_argument("--max-items", type=int, default=0,
@@ -216,10 +325,18 @@ def main():
     out = d_out.numpy()
     if not np.all(np.isfinite(out)):
         raise SystemExit("bench.py: non-finite output")
+    ndft_err = ndft_error(d_img.numpy(), out, wl, rng)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, samples, args.cpu_seconds)
+
+    cgb = None
+    if not args.no_cg:
+        try:
+            cgb = batched_cgsense()
+        except Exception as e:  # reported, never hides the headline number
+            cgb = {"error": f"{type(e).__name__}: {e}"}
 
     line = {
         "metric": METRIC,
@@ -238,6 +355,8 @@ def main():
                    "samples": M, "coils": C, "step": "BACKWARDS_NC2C + FORWARDS_C2NC, DCF both ways"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_cgsense": cgb,
+        "ndft_error": ndft_err,
         "stages_ms_per_step": {k: round(v[0] / args.steps, 5) for k, v in stages.items()},
         "preprocess_ms": round(prep_ms, 2),
     }

@@ -20,6 +20,7 @@ EXPORTED_SYMBOLS = [
     "gt_nufft_convolve",
     "gt_nufft_fft",
     "gt_nufft_deapodize",
+    "gt_nufft_grad_traj",
     "gt_nufft_get_matrix_size",
     "gt_nufft_get_matrix_size_os",
     "gt_nufft_get_W",
@@ -48,6 +49,7 @@ EXPORTED_SYMBOLS = [
     "gt_cgsense_batch_create",
     "gt_cgsense_batch_run",
     "gt_cgsense_batch_destroy",
+    "gt_cgsense_batch_branches",
 ]
 
 # gt_status
@@ -124,6 +126,7 @@ def lib():
         "gt_nufft_convolve": [P, P, SP, I, P, SP, I, I, I, P],
         "gt_nufft_fft": [P, P, SP, I, I, I, P],
         "gt_nufft_deapodize": [P, P, SP, I, I, P],
+        "gt_nufft_grad_traj": [P, P, SP, I, P, SP, I, P, S, P, P],
         "gt_nufft_get_matrix_size": [P, SP],
         "gt_nufft_get_matrix_size_os": [P, SP],
         "gt_nufft_get_W": [P, ctypes.POINTER(D)],
@@ -151,6 +154,7 @@ def lib():
         "gt_cgsense_batch_create": [PP, PP, PP, PP, I, ctypes.POINTER(gt_cg_desc), I, P],
         "gt_cgsense_batch_run": [P, P],
         "gt_cgsense_batch_destroy": [P],
+        "gt_cgsense_batch_branches": [P, ctypes.POINTER(I)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <memory>
 #include <numeric>
@@ -437,6 +438,7 @@ struct gt_cg_batch {
     int device = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int branches = 0;  // chosen by tune_graph
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> events;
     ~gt_cg_batch() {
@@ -449,16 +451,15 @@ struct gt_cg_batch {
 
 namespace {
 
-constexpr int kBranches = 8;  // parallel branches of the batched graph
-
-// Problems longest first onto the least loaded branch (work ~ samples x coils
-// + image x coils).
+// Problems longest first onto the least loaded branch.  Cost of a solve ~
+// samples x coils + grid x coils (bandwidth) plus a fixed term for its ~150
+// launches (small problems are latency bound).
 std::vector<std::vector<int>> assign_branches(const std::vector<gt_sense_op*>& ops, int nbr) {
     std::vector<int> order(ops.size());
     std::iota(order.begin(), order.end(), 0);
     auto work = [&](int i) {
         const gt_sense_op* o = ops[size_t(i)];
-        return double(o->M * o->F * o->C) + 2.25 * double(o->nimg * o->F * o->C);
+        return double(o->M * o->F * o->C) + 2.25 * double(o->nimg * o->F * o->C) + 4.0e5;
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
     std::vector<std::vector<int>> br(static_cast<size_t>(nbr));
@@ -471,24 +472,23 @@ std::vector<std::vector<int>> assign_branches(const std::vector<gt_sense_op*>& o
     return br;
 }
 
-// Captures on a stream of its own (the caller's may be the null stream, which
-// cannot be captured); the instantiated graph is launched on the caller's.
-void build_graph(gt_cg_batch* b) {
-    const int nbr = std::min<int>(kBranches, int(b->ops.size()));
+// Captures one graph with nbr parallel branches, on a stream of its own (the
+// caller's may be the null stream, which cannot be captured): fork from the
+// capture stream, each branch runs its problems' whole solves back to back,
+// join.  The instantiated graph is launched on the caller's stream.
+void build_graph(gt_cg_batch* b, int nbr, hipGraph_t* graph, hipGraphExec_t* exec) {
     auto br = assign_branches(b->ops, nbr);
-    for (int i = 0; i < nbr + 1; i++) {
+    while (int(b->streams.size()) < nbr + 1) {
         hipStream_t st;
         GT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         b->streams.push_back(st);
     }
-    hipStream_t s = b->streams[size_t(nbr)];
-    for (int i = 0; i < nbr + 1; i++) {
+    while (int(b->events.size()) < nbr + 1) {
         hipEvent_t e;
         GT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->events.push_back(e);
     }
-    // fork from s to the branch streams, each branch runs its problems' whole
-    // solves back to back, join back into s
+    hipStream_t s = b->streams[size_t(nbr)];
     GT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     GT_HIP(hipEventRecord(b->events[0], s));
     for (int k = 0; k < nbr; k++) {
@@ -498,8 +498,42 @@ void build_graph(gt_cg_batch* b) {
         GT_HIP(hipEventRecord(b->events[size_t(k) + 1], bs));
         GT_HIP(hipStreamWaitEvent(s, b->events[size_t(k) + 1], 0));
     }
-    GT_HIP(hipStreamEndCapture(s, &b->graph));
-    GT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    GT_HIP(hipStreamEndCapture(s, graph));
+    GT_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+}
+
+// Autotuner: the number of parallel branches that runs the batch fastest.
+// Each candidate graph is built and replayed twice (the second replay timed on
+// the host around a synchronisation); the fastest is kept.
+void tune_graph(gt_cg_batch* b, hipStream_t s) {
+    const int n = int(b->ops.size());
+    std::vector<int> cand;
+    for (int c : {2, 4, 8, 16, 32})
+        if (c <= n) cand.push_back(c);
+    if (cand.empty()) cand.push_back(1);
+    double best = 1e300;
+    for (int nbr : cand) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        build_graph(b, nbr, &g, &e);
+        GT_HIP(hipGraphLaunch(e, s));
+        GT_HIP(hipStreamSynchronize(s));
+        const auto t0 = std::chrono::steady_clock::now();
+        GT_HIP(hipGraphLaunch(e, s));
+        GT_HIP(hipStreamSynchronize(s));
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (t < best) {
+            best = t;
+            if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->graph = g;
+            b->exec = e;
+            b->branches = nbr;
+        } else {
+            (void)hipGraphExecDestroy(e);
+            (void)hipGraphDestroy(g);
+        }
+    }
 }
 
 }  // namespace
@@ -606,7 +640,7 @@ gt_status gt_cgsense_batch_create(gt_cg_batch** out, gt_sense_op* const* ops, co
             b->data.push_back(data[i]);
             b->x.push_back(x[i]);
         }
-        if (mode == GT_CG_BATCHED) build_graph(b.get());
+        if (mode == GT_CG_BATCHED) tune_graph(b.get(), s);
         *out = b.release();
     });
 }
@@ -625,6 +659,13 @@ gt_status gt_cgsense_batch_run(gt_cg_batch* b, gt_stream_t stream) {
 
 gt_status gt_cgsense_batch_destroy(gt_cg_batch* b) {
     return guarded([&] { delete b; });
+}
+
+gt_status gt_cgsense_batch_branches(const gt_cg_batch* b, int* branches) {
+    return guarded([&] {
+        GT_REQUIRE(b && branches, GT_ERR_INVALID_ARGUMENT, "null argument");
+        *branches = b->branches;
+    });
 }
 
 }  // extern "C"

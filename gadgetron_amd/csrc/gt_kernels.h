@@ -246,6 +246,94 @@ __global__ __launch_bounds__(256) void c2nc_gather_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Trajectory gradient of y = dcw * C2NC(grid) (no counterpart in the reference:
+// the derivative-of-Kaiser-Bessel kernel of the differentiable NUFFT).  For
+// sample j and dimension d, with p = (k + 1/2) N_os,
+//   dL/dk[j][d] = N_os[d] * sum_coils Re( conj(gy) * dcw * sum_cells dphi/dp_d * grid )
+// where dphi/dp_d replaces dimension d's weight by its derivative in p and
+// gy is the gradient of the real loss L with respect to the samples in the
+// convention of torch (2 dL/d conj(y)).  A window's edge cells, whose
+// weight jumps to zero, contribute their one-sided derivative.
+// ---------------------------------------------------------------------------
+template <int KW>
+__device__ __forceinline__ void window1d_deriv(float p, const GridGeom& g, int d, bool sign, float (&dw)[KW]) {
+    const float r = g.radius;
+    const int lo = (int)ceilf(p - r);
+    const int hi = (int)floorf(p + r);
+#pragma unroll
+    for (int i = 0; i < KW; i++) {
+        const int c = lo + i;
+        const float u = (float)c - p;
+        const float au = fabsf(u);
+        const bool ok = (c <= hi) && !(au > r);
+        // d phi(|c - p|) / dp = -sign(c - p) phi'(|c - p|)
+        float wt = ok ? kb1_deriv(au, g.nos_f[d], g.one_over_W, g.beta[d]) * (u > 0.0f ? -1.0f : 1.0f) : 0.0f;
+        const int cw = wrap_cell(c, g.nos[d]);
+        if (sign && (cw & 1)) wt = -wt;
+        dw[i] = wt;
+    }
+}
+
+template <int D, int KW>
+__global__ __launch_bounds__(256) void traj_grad_kernel(const float2* __restrict__ grid,
+                                                        const float2* __restrict__ gy,
+                                                        const float* __restrict__ p, int M, int F, int C,
+                                                        GridGeom g, int sign, const float* __restrict__ dcw,
+                                                        long long dcw_len, float* __restrict__ out) {
+    const long long nsamp = (long long)M * F;
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nsamp) return;
+    const int f = (int)(j / M);
+    float w[3][KW], dw[3][KW];
+    int cell[3][KW];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d < D) {
+            window1d<KW>(p[j * D + d], g, d, sign != 0, w[d], cell[d]);
+            window1d_deriv<KW>(p[j * D + d], g, d, sign != 0, dw[d]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < KW; i++) {
+                w[d][i] = i == 0 ? 1.0f : 0.0f;
+                dw[d][i] = 0.0f;
+                cell[d][i] = 0;
+            }
+        }
+    }
+    const long long G = g.G;
+    const long long FG = (long long)F * G;
+    const float2* gb = grid + (long long)f * G;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    const int nz = D >= 3 ? KW : 1, ny = D >= 2 ? KW : 1;
+    for (int c = 0; c < C; c++) {
+        float sr[3] = {0.0f, 0.0f, 0.0f}, si[3] = {0.0f, 0.0f, 0.0f};
+        for (int iz = 0; iz < nz; iz++) {
+            for (int iy = 0; iy < ny; iy++) {
+#pragma unroll
+                for (int ix = 0; ix < KW; ix++) {
+                    const long long cl = cell[0][ix] + (long long)g.nos[0] * (cell[1][iy] + (long long)g.nos[1] * cell[2][iz]);
+                    const float wx = w[0][ix], wy = w[1][iy], wz = w[2][iz];
+                    const float gx = dw[0][ix] * wy * wz, gyy = wx * dw[1][iy] * wz, gz = wx * wy * dw[2][iz];
+                    if (gx == 0.0f && gyy == 0.0f && gz == 0.0f) continue;
+                    const float2 v = gb[(long long)c * FG + cl];
+                    sr[0] += gx * v.x;  si[0] += gx * v.y;
+                    sr[1] += gyy * v.x; si[1] += gyy * v.y;
+                    sr[2] += gz * v.x;  si[2] += gz * v.y;
+                }
+            }
+        }
+        const long long o = (long long)c * nsamp + j;
+        const float2 gv = gy[o];
+        const float s = dcw ? dcw[o % dcw_len] : 1.0f;
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc[d] += s * (gv.x * sr[d] + gv.y * si[d]);  // Re(conj(gy) S)
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        if (d < D) out[j * D + d] = g.nos_f[d] * acc[d];
+}
+
+// ---------------------------------------------------------------------------
 // NC2C, deterministic tile gather.  One wavefront per work item owns one tile
 // of the oversampled grid (one frame, a chunk of coils) in LDS.  Its lanes
 // take the tile's samples in ascending sample order, 64 at a time, and add

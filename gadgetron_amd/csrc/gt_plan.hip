@@ -451,6 +451,16 @@ struct LaunchC2NC {
 };
 
 template <int D, int KW>
+struct LaunchTrajGrad {
+    static void run(const float2* grid, const float2* gy, const float* pos, int M, int F, int C,
+                    const gt::GridGeom& g, const float* dcw, long long dcw_len, float* out, hipStream_t s) {
+        const long long n = (long long)M * F;
+        hipLaunchKernelGGL((gt::traj_grad_kernel<D, KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, grid,
+                           gy, pos, M, F, C, g, 1, dcw, dcw_len, out);
+    }
+};
+
+template <int D, int KW>
 struct LaunchNC2CTile {
     static void run(const float2* samples, float2* grid, float2* partials, const float* pos,
                     const uint32_t* tile_samples, const gt::WorkItem* items, int n_items, int M,
@@ -1366,6 +1376,31 @@ gt_status gt_nufft_deapodize(gt_nufft_plan* p, void* image, const size_t* dims, 
         const long long total = (long long)(sh.nb * p->G());
         hipLaunchKernelGGL(gt::crop_deapod_kernel, dim3(grid_blocks(total)), dim3(256), 0, S(stream),
                            (const float2*)image, (float2*)image, (long long)sh.nb, p->geom, pc);
+        GT_HIP(hipGetLastError());
+    });
+}
+
+gt_status gt_nufft_grad_traj(gt_nufft_plan* p, const void* image, const size_t* dims, int ndim,
+                             const void* grad_samples, const size_t* sdims, int sndim, const float* dcw,
+                             size_t dcw_len, float* grad_traj, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(p && image && grad_samples && grad_traj, GT_ERR_INVALID_ARGUMENT, "null argument");
+        GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
+        GT_REQUIRE(p->M > 0 && p->F > 0, GT_ERR_NOT_PREPROCESSED, "plan has not been preprocessed");
+        Shape sh = image_shape(p, dims, ndim, true);
+        check_batches(p, sh, elements(sdims, sndim));
+        set_device(p);
+        hipStream_t s = S(stream);
+        // the grid of FORWARDS_C2NC, recomputed from the image (not stored by
+        // the forward pass): pad, deapodize, centred FFT, caller layout
+        const size_t G = p->G();
+        const float scale = centring_global_sign(p) / std::sqrt(float(G));
+        p->grid.reserve(sh.nb * G * sizeof(float2));
+        pad_deapod(p, image, p->grid.ptr, sh.nb, sh.oversampled, scale, s);
+        run_fft(p, p->grid.ptr, sh.nb, true, s);
+        const long long dl = (long long)std::max<size_t>(dcw_len, 1);
+        dispatch<LaunchTrajGrad>(p->D, p->KW, (const float2*)p->grid.ptr, (const float2*)grad_samples,
+                                 p->traj.as<float>(), int(p->M), int(p->F), int(sh.C), p->geom, dcw, dl, grad_traj, s);
         GT_HIP(hipGetLastError());
     });
 }

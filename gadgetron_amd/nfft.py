@@ -247,5 +247,13 @@ class NFFTPlan:
         check(lib().gt_nufft_deapodize(self._h, _ptr(image), sizes(d), len(d), int(bool(fourier_domain)),
                                        _stream(stream)))
 
+    def grad_traj(self, image, grad_samples, grad_traj, dcw=None, stream=None):
+        """dL/dk of y = compute(image, FORWARDS_C2NC, dcw) given grad_samples =
+        2 dL/d conj(y) (torch's convention); grad_traj has the trajectory's shape."""
+        di, ds = _gt_dims(image), _gt_dims(grad_samples)
+        dl = 0 if dcw is None else _numel(dcw)
+        check(lib().gt_nufft_grad_traj(self._h, _ptr(image), sizes(di), len(di), _ptr(grad_samples), sizes(ds),
+                                       len(ds), _ptr(dcw), dl, _ptr(grad_traj), _stream(stream)))
+
     def synchronize(self, stream=None):
         check(lib().gt_stream_synchronize(_stream(stream)))

@@ -103,6 +103,13 @@ class CgBatch:
     def run(self, stream=None):
         check(lib().gt_cgsense_batch_run(self._h, _stream(stream)))
 
+    @property
+    def branches(self):
+        """Parallel graph branches the autotuner chose (0 for the plain mode)."""
+        n = ctypes.c_int()
+        check(lib().gt_cgsense_batch_branches(self._h, ctypes.byref(n)))
+        return n.value
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):

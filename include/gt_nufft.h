@@ -108,6 +108,16 @@ gt_status gt_nufft_fft(gt_nufft_plan* plan, void* data, const size_t* dims, int 
 gt_status gt_nufft_deapodize(gt_nufft_plan* plan, void* image, const size_t* dims, int ndim,
                              int fourier_domain, gt_stream_t stream);
 
+/* Trajectory gradient of the differentiable forward NUFFT (no counterpart in
+ * the reference: the derivative-of-Kaiser-Bessel kernel behind the torch
+ * autograd function).  For y = compute(image, FORWARDS_C2NC, dcw) and a real
+ * loss L with grad_samples = 2 dL/d conj(y) ([M, F, C], torch's convention),
+ * writes grad_traj = dL/dk, [M, F] of D floats (the trajectory's layout).  The
+ * oversampled grid of the forward pass is recomputed from image. */
+gt_status gt_nufft_grad_traj(gt_nufft_plan* plan, const void* image, const size_t* dims, int ndim,
+                             const void* grad_samples, const size_t* sdims, int sndim, const float* dcw,
+                             size_t dcw_len, float* grad_traj, gt_stream_t stream);
+
 /* NFFT_plan getters (NFFT.h:135-151) and preprocess results. */
 gt_status gt_nufft_get_matrix_size(const gt_nufft_plan* plan, size_t* N);
 gt_status gt_nufft_get_matrix_size_os(const gt_nufft_plan* plan, size_t* N_os);
