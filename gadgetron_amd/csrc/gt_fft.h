@@ -222,16 +222,25 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_pad_kernel(
     const float sy = fg.scale * fg.fy[y] * ((y & 1) ? -1.0f : 1.0f);
     const float2* in = img + ((long long)b * fg.Ny + iy) * fg.Nx;
     float2* sb = &sbuf[rl][0];
-    for (int x = t; x < OX; x += kFftRowThreads) {
+    // every load of the row before the first LDS store (one memory latency)
+    constexpr int kPer = (OX + kFftRowThreads - 1) / kFftRowThreads;
+    float2 v[kPer];
+    float fxs[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+        const int x = t + i * kFftRowThreads;
         const int ix = x - fg.offx;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (live && ix >= 0 && ix < fg.Nx) {
-            v = in[ix];
-            const float f = sy * fg.fx[x] * ((x & 1) ? -1.0f : 1.0f);
-            v.x *= f;
-            v.y *= f;
+        const bool ok = live && x < OX && ix >= 0 && ix < fg.Nx;
+        v[i] = ok ? in[ix] : make_float2(0.0f, 0.0f);
+        fxs[i] = x < OX ? fg.fx[x] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+        const int x = t + i * kFftRowThreads;
+        if (x < OX) {
+            const float f = sy * fxs[i] * ((x & 1) ? -1.0f : 1.0f);
+            sb[fft_slot<true>(x, 1)] = make_float2(v[i].x * f, v[i].y * f);
         }
-        sb[fft_slot<true>(x, 1)] = v;
     }
     __syncthreads();
     fft_passes<OX, 1, kFftRowThreads, true>(sb, 1, t, stw, s);
@@ -258,8 +267,20 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_crop_kernel
     const int y = iy + fg.offy;
     const float2* in = grid + ((long long)y * fg.nb + b) * OX;
     float2* sb = &sbuf[rl][0];
-    for (int x = t; x < OX; x += kFftRowThreads)
-        sb[fft_slot<true>(x, 1)] = live ? in[x] : make_float2(0.0f, 0.0f);
+    {
+        constexpr int kPer = (OX + kFftRowThreads - 1) / kFftRowThreads;
+        float2 v[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int x = t + i * kFftRowThreads;
+            v[i] = (live && x < OX) ? in[x] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int x = t + i * kFftRowThreads;
+            if (x < OX) sb[fft_slot<true>(x, 1)] = v[i];
+        }
+    }
     __syncthreads();
     fft_passes<OX, 1, kFftRowThreads, true>(sb, 1, t, stw, s);
     if (!live) return;
@@ -294,10 +315,18 @@ __global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid
     const long long rs = (long long)fg.nb * Ox;  // grid row stride
     float2* col = grid + (long long)b * Ox + x;
     const int ylo = fg.offy, yhi = fg.offy + fg.Ny;
-    for (int y = yr; y < OY; y += TPS) {
-        float2 v = make_float2(0.0f, 0.0f);
-        if (xin && (!BAND_IN || (y >= ylo && y < yhi))) v = col[y * rs];
-        sbuf[y * W + xl] = v;
+    {
+        // every load of the column block before the first LDS store
+        static_assert(OY % TPS == 0, "column length must be a multiple of the threads per column");
+        constexpr int kPer = OY / TPS;
+        float2 v[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int y = yr + i * TPS;
+            v[i] = (xin && (!BAND_IN || (y >= ylo && y < yhi))) ? col[y * rs] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; i++) sbuf[(yr + i * TPS) * W + xl] = v[i];
     }
     __syncthreads();
     fft_passes<OY, 1, TPS, false>(&sbuf[xl], W, yr, stw, s);

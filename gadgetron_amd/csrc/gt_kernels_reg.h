@@ -85,23 +85,25 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
 #pragma unroll
     for (int i = 0; i < kRT * kRT; i++) acc[i] = 0.0f;
 
-    // Pipeline: while chunk c is gridded from LDS buffer c & 1, the samples of
-    // chunk c + 1 are in flight and the table of chunk c + 2 is in registers.
-    constexpr int kDV = DM == 2 ? kLoadCoils : 1;
+    // Pipeline, per chunk c: the samples of chunk c (in registers) move to the
+    // LDS transpose, the table of chunk c + 1 moves to LDS buffer (c + 1) & 1,
+    // the table of chunk c + 2 and the samples of chunk c + 1 are issued, and
+    // chunk c is gridded from LDS while those loads are in flight.
+    constexpr int kDV = DM == 2 ? kLoadCoils : 1;  // dcw per loaded coil (DM 2) or per entry (DM 1)
     float4 wreg;
     uint32_t hreg;
-    float2 svA[kLoadCoils], svB[kLoadCoils];
-    float dvA[kDV], dvB[kDV];
+    float2 sv[kLoadCoils];
+    float dv[kDV];
 #pragma unroll
-    for (int u = 0; u < kLoadCoils; u++) svA[u] = svB[u] = make_float2(0.0f, 0.0f);
+    for (int u = 0; u < kLoadCoils; u++) sv[u] = make_float2(0.0f, 0.0f);
 #pragma unroll
-    for (int u = 0; u < kDV; u++) dvA[u] = dvB[u] = 1.0f;
+    for (int u = 0; u < kDV; u++) dv[u] = 1.0f;
 
     auto stage = [&](int b) {
         sw[b][lane] = wreg;
         if (lane < 2 * kNK) sh[b][lane] = hreg;
     };
-    auto issue = [&](int b, float2 (&sv)[kLoadCoils], float (&dv)[kDV]) {
+    auto issue = [&](int b) {
         const uint32_t j = sh[b][2 * kk];
 #pragma unroll
         for (int u = 0; u < kLoadCoils; u++) {
@@ -110,33 +112,42 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
             sv[u] = s2[(long long)c * FM + j];
             if constexpr (DM == 2) dv[u] = dcw[dcw_index(c, j, FM, dcw_len)];
         }
+        if constexpr (DM == 1) dv[0] = dcw[j];
     };
-    auto grid_chunk = [&](int b, const float2 (&sv)[kLoadCoils], const float (&dv)[kDV]) {
-        // redistribute the chunk's samples: lane = (coil, re/im) reads entry k
+    auto transpose = [&]() {
+        // lane = (entry, coil quarter) -> [entry][coil][re, im]; the gridding
+        // reads lane = (coil, re/im) of entry k
 #pragma unroll
         for (int u = 0; u < kLoadCoils; u++) {
             float2 v = sv[u];
-            if constexpr (DM == 2) {
-                v.x *= dv[u];
-                v.y *= dv[u];
+            if constexpr (DM != 0) {
+                const float d = dv[DM == 2 ? u : 0];
+                v.x *= d;
+                v.y *= d;
             }
             *reinterpret_cast<float2*>(&xs[kk * kXsRow + 2 * (cq * kLoadCoils + u)]) = v;
         }
+    };
+    auto grid_chunk = [&](int b) {
         const int hv = (int)sh[b][lane & 31];
-        float dk[kNK];
+        // entry k + 1's weights and sample are read while entry k is gridded
+        float4 n0 = sw[b][0], n1 = sw[b][1], n2 = sw[b][2], n3 = sw[b][3];
+        float nsk = xs[lane];
 #pragma unroll
         for (int k = 0; k < kNK; k++) {
-            if constexpr (DM == 1) dk[k] = dcw[(uint32_t)__builtin_amdgcn_readlane(hv, 2 * k)];
-            else dk[k] = 1.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < kNK; k++) {
+            const float4 x0 = n0, x1 = n1, y0 = n2, y1 = n3;
+            const float sk = nsk;
+            if (k + 1 < kNK) {
+                n0 = sw[b][4 * (k + 1)];
+                n1 = sw[b][4 * (k + 1) + 1];
+                n2 = sw[b][4 * (k + 1) + 2];
+                n3 = sw[b][4 * (k + 1) + 3];
+                nsk = xs[(k + 1) * kXsRow + lane];
+            }
             const uint32_t rows = (uint32_t)__builtin_amdgcn_readlane(hv, 2 * k + 1);
-            const float4 x0 = sw[b][4 * k], x1 = sw[b][4 * k + 1], y0 = sw[b][4 * k + 2], y1 = sw[b][4 * k + 3];
             const float wc[kRT] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
             const float wr[kRT] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
-            const float sk = xs[k * kXsRow + lane];
-            const float s = DM == 1 ? sk * dk[k] : sk;
+            const float s = sk;  // dcw already applied in the transpose
             // rows: bits 0-7 covered tile rows (a scalar test per row); all 8
             // columns are updated (zero weight outside the window): skipping
             // half-rows costs more in branches than the FMAs it saves
@@ -156,18 +167,14 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
         nc2c_chunk_load(wt, hd, it.begin, it.end, lane, wreg, hreg);
         stage(0);
         nc2c_chunk_load(wt, hd, it.begin + kNK, it.end, lane, wreg, hreg);
-        issue(0, svA, dvA);
-        for (uint32_t c = 0;; c += 2) {
-            stage(1);
+        issue(0);
+        for (uint32_t c = 0; c < nch; c++) {
+            const int b = (int)(c & 1u);
+            transpose();
+            stage(b ^ 1);
             nc2c_chunk_load(wt, hd, it.begin + (c + 2) * kNK, it.end, lane, wreg, hreg);
-            issue(1, svB, dvB);
-            grid_chunk(0, svA, dvA);
-            if (c + 1 >= nch) break;
-            stage(0);
-            nc2c_chunk_load(wt, hd, it.begin + (c + 3) * kNK, it.end, lane, wreg, hreg);
-            issue(0, svA, dvA);
-            grid_chunk(1, svB, dvB);
-            if (c + 2 >= nch) break;
+            issue(b ^ 1);
+            grid_chunk(b);
         }
     }
 
