@@ -1,0 +1,489 @@
+// gt_kernels_3d.h -- the 3D gridding kernels of the hot path.
+//
+// nc2c_reg3d_kernel (NFFT_H_convolve_kernel, ConvolverNC2C_standard.cuh:298-355)
+//   One wavefront owns one 8 x 8 x 4 tile of the grid for 8 coils.  Lane
+//   (plane z, coil, real or imaginary part) holds the tile's 8 x 8 cells of
+//   plane z in 64 registers.  A table entry carries the sample's weights for
+//   the 8 tile columns, 8 tile rows and 4 tile planes (zero outside the
+//   window) and the mask of rows the window covers; a sample costs one
+//   multiply by the lane's plane weight and 8 FMAs per covered row.  Tables
+//   and samples stream through LDS two chunks ahead as in the 2D kernel, and
+//   each lane sums its cells in table order (bitwise repeatable).
+//
+// c2nc_multi3d_kernel (NFFT_convolve_kernel, ConvolverC2NC.cuh:260-307)
+//   A workgroup stages one 4 x 4 x 4 tile plus the kernel halo for 8 coils in
+//   LDS, cell-major with the 16 (coil, re/im) floats of a cell contiguous.
+//   Each wavefront then gathers two samples at a time: lane (sample slot,
+//   x half, coil, re/im) sums its half of every x row of the window, and the
+//   two halves are added across lanes.  A read instruction covers two runs
+//   of 16 consecutive floats per sample, so the window reads are free of
+//   bank conflicts.
+#pragma once
+
+#include "gt_kernels_reg.h"
+
+namespace gt {
+
+// ---- NC2C ------------------------------------------------------------------
+constexpr int k3T0 = 8, k3T1 = 8, k3T2 = 4;  // register tile (x, y, z)
+constexpr int k3Coils = 8;                   // coils per wavefront (4 planes x 8 coils x 2)
+constexpr int k3NW = 20;                     // weight floats per table entry: 8 x, 8 y, 4 z
+constexpr int k3NK = 16;                     // table entries per chunk moved through LDS
+constexpr int k3XsRow = 2 * k3Coils + 2;     // floats per entry of the sample transpose
+constexpr int k3LoopFloats = 2 * k3NK * k3NW + 2 * 2 * k3NK + k3NK * k3XsRow;
+constexpr int k3OutFloats = 32 * kSubP;      // write-out: half a plane tile (32 cells) x 64 lanes
+constexpr int k3NC2CLdsFloats = ((k3OutFloats > k3LoopFloats ? k3OutFloats : k3LoopFloats) + 3) & ~3;
+static_assert(k3NK * k3NW / 4 <= 2 * kLanes, "a chunk's weights load as at most two float4 per lane");
+static_assert(k3T2 * 2 * k3Coils == kLanes, "lanes = planes x coils x (re, im)");
+
+// Weight of grid cell gc (wrapped coordinate) for a sample at p along dim d:
+// the window's unwrapped cell that wraps to gc, if any, weighted as in
+// window_raw (the window is never longer than the grid).
+__device__ __forceinline__ float cell_weight(float p, const GridGeom& g, int d, int gc) {
+    const float r = g.radius;
+    const int lo = (int)ceilf(p - r);
+    const int hi = (int)floorf(p + r);
+    const int c = lo + wrap_cell(gc - lo, g.nos[d]);
+    if (c > hi) return 0.0f;
+    const float u = fabsf((float)c - p);
+    return u > r ? 0.0f : kb1(u, g.nos_f[d], g.one_over_W, g.beta[d]);
+}
+
+__device__ __forceinline__ void tile_origin3(int tile, const GridGeom& g, int (&org)[3]) {
+    org[0] = (tile % g.ntiles[0]) * g.tile[0];
+    tile /= g.ntiles[0];
+    org[1] = (tile % g.ntiles[1]) * g.tile[1];
+    org[2] = (tile / g.ntiles[1]) * g.tile[2];
+}
+
+// NC2C table, per (tile, sample) pair e, in two arrays:
+//   wt[20 e + c]   weight of tile column c (c < 8), row c - 8 (c < 16), plane c - 16
+//   hd[2 e]        sample index j
+//   hd[2 e + 1]    bits 0-7: tile rows the window covers
+__global__ void nc2c_table3d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list,
+                                    const uint32_t* __restrict__ keys, long long n, GridGeom g,
+                                    float* __restrict__ wt, uint32_t* __restrict__ hd) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t j = list[e];
+    int org[3];
+    tile_origin3((int)(keys[e] % (uint32_t)g.ntiles_total), g, org);
+    const float px = p[(long long)j * 3 + 0], py = p[(long long)j * 3 + 1], pz = p[(long long)j * 3 + 2];
+    float w[k3NW];
+    uint32_t rows = 0;
+#pragma unroll
+    for (int c = 0; c < k3T0; c++) w[c] = cell_weight(px, g, 0, org[0] + c);
+#pragma unroll
+    for (int c = 0; c < k3T1; c++) {
+        w[k3T0 + c] = cell_weight(py, g, 1, org[1] + c);
+        rows |= (w[k3T0 + c] != 0.0f ? 1u : 0u) << c;
+    }
+#pragma unroll
+    for (int c = 0; c < k3T2; c++) w[k3T0 + k3T1 + c] = cell_weight(pz, g, 2, org[2] + c);
+    float4* t = reinterpret_cast<float4*>(wt + e * k3NW);
+#pragma unroll
+    for (int q = 0; q < k3NW / 4; q++) t[q] = make_float4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    hd[2 * e] = j;
+    hd[2 * e + 1] = rows;
+}
+
+// Loads chunk entries [cb, cb + k3NK) of the table (entries [begin, end), end >
+// begin): float4 f of the chunk's weights in lane f (and lane f - 64 for f >=
+// 64), the head dwords in lanes 0..31.  Entries past `end` read as zeros
+// (sample 0 with no rows).  Addresses are clamped so that every call issues
+// the same loads.
+__device__ __forceinline__ void nc2c3_chunk_load(const float* __restrict__ wt, const uint32_t* __restrict__ hd,
+                                                 uint32_t cb, uint32_t end, int lane, float4& w0, float4& w1,
+                                                 uint32_t& h) {
+    constexpr int kF4 = k3NW / 4;  // float4 per entry
+    const float4* w4 = reinterpret_cast<const float4*>(wt);
+    const uint32_t e0 = cb + (uint32_t)(lane / kF4);
+    const float4 a = w4[(size_t)min(e0, end - 1) * kF4 + lane % kF4];
+    w0 = e0 < end ? a : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int f1 = kLanes + (lane & 15);  // lanes 16..63 repeat lanes 0..15 (only 0..15 are stored)
+    const uint32_t e1 = cb + (uint32_t)(f1 / kF4);
+    const float4 b = w4[(size_t)min(e1, end - 1) * kF4 + f1 % kF4];
+    w1 = e1 < end ? b : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t eh = cb + (uint32_t)((lane & 31) >> 1);
+    const uint32_t hv = hd[(size_t)min(eh, end - 1) * 2 + (lane & 1)];
+    h = eh < end ? hv : 0u;
+}
+
+// DM: how dcw is applied (dcw_mode): 0 none, 1 per sample, 2 per coil element.
+template <int DM>
+__global__ __launch_bounds__(64, 3) void nc2c_reg3d_kernel(
+    const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
+    const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
+    int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
+    int accumulate) {
+    __shared__ float4 smem[k3NC2CLdsFloats / 4];
+    float* xp = reinterpret_cast<float*>(smem);                                  // write-out transpose
+    float (*sw)[k3NK * k3NW] = reinterpret_cast<float (*)[k3NK * k3NW]>(smem);  // weights of two chunks
+    uint32_t (*sh)[2 * k3NK] = reinterpret_cast<uint32_t (*)[2 * k3NK]>(reinterpret_cast<float*>(smem) + 2 * k3NK * k3NW);
+    float* xs = reinterpret_cast<float*>(smem) + 2 * k3NK * k3NW + 2 * 2 * k3NK;  // [entry][coil][re, im]
+    const int lane = threadIdx.x;
+    const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
+    const int c0 = blockIdx.y * k3Coils;
+    const long long FM = (long long)F * M;
+    const uint32_t nch = (it.end - it.begin + k3NK - 1) / k3NK;
+    // sample loads: lane = (entry kk, coil quarter cq), 2 coils each
+    constexpr int kLC = k3Coils / 4;
+    const int kk = lane & (k3NK - 1), cq = lane >> 4;
+    const float2* s2 = reinterpret_cast<const float2*>(samples);
+    // gridding: lane = (plane zp, coil, re/im)
+    const int zp = lane >> 4, q16 = lane & 15;
+
+    float acc[k3T0 * k3T1];
+#pragma unroll
+    for (int i = 0; i < k3T0 * k3T1; i++) acc[i] = 0.0f;
+
+    constexpr int kDV = DM == 2 ? kLC : 1;
+    float4 w0, w1;
+    uint32_t hreg;
+    float2 sv[kLC];
+    float dv[kDV];
+#pragma unroll
+    for (int u = 0; u < kLC; u++) sv[u] = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < kDV; u++) dv[u] = 1.0f;
+
+    auto stage = [&](int b) {
+        float4* d = reinterpret_cast<float4*>(sw[b]);
+        d[lane] = w0;
+        if (lane < k3NK * k3NW / 4 - kLanes) d[kLanes + lane] = w1;
+        if (lane < 2 * k3NK) sh[b][lane] = hreg;
+    };
+    auto issue = [&](int b) {
+        const uint32_t j = sh[b][2 * kk];
+#pragma unroll
+        for (int u = 0; u < kLC; u++) {
+            int c = c0 + cq * kLC + u;
+            c = c < C ? c : c0;  // coils past the last one repeat a valid coil, never stored
+            sv[u] = s2[(long long)c * FM + j];
+            if constexpr (DM == 2) dv[u] = dcw[dcw_index(c, j, FM, dcw_len)];
+        }
+        if constexpr (DM == 1) dv[0] = dcw[j];
+    };
+    auto transpose = [&]() {
+#pragma unroll
+        for (int u = 0; u < kLC; u++) {
+            float2 v = sv[u];
+            if constexpr (DM != 0) {
+                const float d = dv[DM == 2 ? u : 0];
+                v.x *= d;
+                v.y *= d;
+            }
+            *reinterpret_cast<float2*>(&xs[kk * k3XsRow + 2 * (cq * kLC + u)]) = v;
+        }
+    };
+    auto grid_chunk = [&](int b) {
+        const int hv = (int)sh[b][lane & 31];
+        const float* wb = sw[b];
+#pragma unroll
+        for (int k = 0; k < k3NK; k++) {
+            const float4 x0 = *reinterpret_cast<const float4*>(wb + k * k3NW);
+            const float4 x1 = *reinterpret_cast<const float4*>(wb + k * k3NW + 4);
+            const float4 y0 = *reinterpret_cast<const float4*>(wb + k * k3NW + 8);
+            const float4 y1 = *reinterpret_cast<const float4*>(wb + k * k3NW + 12);
+            const float wz = wb[k * k3NW + 16 + zp];
+            const float s = xs[k * k3XsRow + q16] * wz;  // dcw already applied in the transpose
+            const uint32_t rows = (uint32_t)__builtin_amdgcn_readlane(hv, 2 * k + 1);
+            const float wc[k3T0] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            const float wr[k3T1] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+#pragma unroll
+            for (int cy = 0; cy < k3T1; cy++) {
+                if (!((rows >> cy) & 1u)) continue;
+                const float sy = s * wr[cy];
+#pragma unroll
+                for (int cx = 0; cx < k3T0; cx++) acc[cy * k3T0 + cx] = fmaf(sy, wc[cx], acc[cy * k3T0 + cx]);
+            }
+        }
+    };
+
+    if (nch > 0) {
+        nc2c3_chunk_load(wt, hd, it.begin, it.end, lane, w0, w1, hreg);
+        stage(0);
+        nc2c3_chunk_load(wt, hd, it.begin + k3NK, it.end, lane, w0, w1, hreg);
+        issue(0);
+        for (uint32_t c = 0; c < nch; c++) {
+            const int b = (int)(c & 1u);
+            transpose();
+            stage(b ^ 1);
+            nc2c3_chunk_load(wt, hd, it.begin + (c + 2) * k3NK, it.end, lane, w0, w1, hreg);
+            issue(b ^ 1);
+            grid_chunk(b);
+        }
+    }
+
+    // write-out through LDS, half a plane tile (4 rows of 8 cells) per pass:
+    // lanes run along x within (plane, coil, y) rows for coalesced stores
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    int org[3];
+    tile_origin3((int)(it.key % (uint32_t)g.ntiles_total), g, org);
+    const int lr = lane >> 4, li = lane & 15;
+    const int lx = li >> 1, comp = li & 1;
+    const int gx = org[0] + lx;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 32; c++) xp[c * kSubP + lane] = acc[half * 32 + c];
+        __syncthreads();
+        // rows r = (plane, coil, y in half): 4 x 8 x 4 = 128, four per pass
+        for (int r = lr; r < k3T2 * k3Coils * 4; r += 4) {
+            const int ly4 = r & 3, cl = (r >> 2) & (k3Coils - 1), pz = r >> 5;
+            const int ly = half * 4 + ly4;
+            const int cc = c0 + cl;
+            const int gy = org[1] + ly, gz = org[2] + pz;
+            if (cc >= C || gx >= g.nos[0] || gy >= g.nos[1] || gz >= g.nos[2]) continue;
+            float v = xp[(ly4 * k3T0 + lx) * kSubP + pz * 16 + 2 * cl + comp];
+            if (it.partial >= 0) {
+                const int cell = (pz * k3T1 + ly) * k3T0 + lx;
+                partials[(((long long)it.partial * C + cc) * (k3T0 * k3T1 * k3T2) + cell) * 2 + comp] = v;
+                continue;
+            }
+            if (sign && ((gx + gy + gz) & 1)) v = -v;
+            const long long o =
+                2 * ((long long)gx + L.sy * gy + L.sz * gz + L.sb * ((long long)cc * F + f)) + comp;
+            if (accumulate) v += grid[o];
+            grid[o] = v;
+        }
+    }
+}
+
+// Sum of the partial tiles of split tiles in segment order; one thread per
+// (coil, cell, re/im) element of a tile.
+__global__ __launch_bounds__(256) void nc2c_reduce3d_kernel(const float* __restrict__ partials,
+                                                            float* __restrict__ grid,
+                                                            const RedItem* __restrict__ red, int F, int C,
+                                                            GridGeom g, Layout L, int sign, int accumulate) {
+    constexpr int kCells = k3T0 * k3T1 * k3T2;
+    constexpr int kTileFloats = kCells * 2;
+    const RedItem it = red[blockIdx.x];
+    const int i = blockIdx.y * blockDim.x + threadIdx.x;
+    if (i >= C * kTileFloats) return;
+    const int comp = i & 1, cell = (i >> 1) % kCells, cc = i / kTileFloats;
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    int org[3];
+    tile_origin3((int)(it.key % (uint32_t)g.ntiles_total), g, org);
+    const int gx = org[0] + cell % k3T0;
+    const int gy = org[1] + (cell / k3T0) % k3T1;
+    const int gz = org[2] + cell / (k3T0 * k3T1);
+    if (gx >= g.nos[0] || gy >= g.nos[1] || gz >= g.nos[2]) return;
+    const long long stride = (long long)C * kTileFloats;
+    const float* src = partials + (long long)it.first * stride + i;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    int s = 0;
+    for (; s + 4 <= it.count; s += 4) {
+        a0 += src[(s + 0) * stride];
+        a1 += src[(s + 1) * stride];
+        a2 += src[(s + 2) * stride];
+        a3 += src[(s + 3) * stride];
+    }
+    for (; s < it.count; s++) a0 += src[s * stride];
+    float v = (a0 + a1) + (a2 + a3);
+    if (sign && ((gx + gy + gz) & 1)) v = -v;
+    const long long o = 2 * ((long long)gx + L.sy * gy + L.sz * gz + L.sb * ((long long)cc * F + f)) + comp;
+    if (accumulate) v += grid[o];
+    grid[o] = v;
+}
+
+// ---- C2NC ------------------------------------------------------------------
+constexpr int k3S0 = 4, k3S1 = 4, k3S2 = 4;  // staged tile (x, y, z), plus the kernel halo
+constexpr int k3CDw = 24;                    // dwords per C2NC table entry
+constexpr int k3XSlots = 8;                  // x weight slots (two halves of up to 4)
+constexpr int k3Seg = 64;                    // entries per C2NC work item (its table is staged)
+constexpr int k3Waves = 8;                   // wavefronts per C2NC workgroup
+constexpr int k3Cell = 2 * k3Coils;          // LDS floats per staged cell
+
+// C2NC table entry (24 dwords):
+//   [0] sample index j
+//   [1] bits 0-7 bx, 8-15 by, 16-23 bz: window slot 0 in the staged tile
+//   [2..9] x weights of slots 0..7, [10..16] y weights of slots 0..6,
+//   [17..23] z weights of slots 0..6 (zero past the window)
+template <int KW>
+__global__ void c2nc_table3d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list,
+                                    long long n, GridGeom g, uint32_t* __restrict__ tab) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t j = list[e];
+    const int fr = (int)floorf(g.radius);
+    uint32_t* t = tab + e * k3CDw;
+    uint32_t packed = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        float pw;
+        const int td = home_tile_1d(p[(long long)j * 3 + d], g, d, pw);
+        float w[KW];
+        int lo;
+        window_raw<KW>(pw, g, d, w, lo);
+        // 0 <= b <= tile for every home sample; the clamp keeps reads inside LDS regardless
+        const int b = min(max(lo - (td * g.tile[d] - fr), 0), g.tile[d]);
+        packed |= (uint32_t)b << (8 * d);
+        const int base = d == 0 ? 2 : (d == 1 ? 10 : 17);
+        const int ns = d == 0 ? k3XSlots : kTabKW;
+#pragma unroll
+        for (int i = 0; i < ns; i++) t[base + i] = __float_as_uint(i < KW ? w[i < KW ? i : 0] : 0.0f);
+    }
+    t[0] = j;
+    t[1] = packed;
+}
+
+// Staged cells: E0 x E1 x E2 (tile + KW - 1 per dimension), then zeroed pad
+// cells: a window reads up to one cell, row and plane past the staged tile
+// (and, with the x halves, 2 XH - KW more cells), always with weight 0.
+__host__ __device__ constexpr int c2nc3_cells(int KW) { return (k3S0 + KW - 1) * (k3S1 + KW - 1) * (k3S2 + KW - 1); }
+__host__ __device__ constexpr int c2nc3_pad_cells(int KW) {
+    return (k3S0 + KW - 1) * (k3S1 + KW - 1) + (k3S0 + KW - 1) + 8;
+}
+__host__ __device__ constexpr int c2nc3_lds_floats(int KW) { return (c2nc3_cells(KW) + c2nc3_pad_cells(KW)) * k3Cell; }
+__host__ __device__ constexpr int c2nc3_lds_bytes(int KW) { return (c2nc3_lds_floats(KW) + k3Seg * k3CDw + k3Seg) * 4; }
+
+// MODE 0: no dcw, no accumulation; 1: one dcw weight per sample (staged with
+// the table), no accumulation; 2: anything else.
+template <int KW, int MODE>
+__global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
+    const float* __restrict__ grid, float* __restrict__ out, const uint32_t* __restrict__ tab,
+    const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g, Layout L, int sign,
+    const float* __restrict__ dcw, long long dcw_len, int accumulate) {
+    extern __shared__ float sg[];  // staged grid, then the item's table, then its dcw
+    constexpr int E0 = k3S0 + KW - 1, E1 = k3S1 + KW - 1;
+    constexpr int NCELL = c2nc3_cells(KW);
+    constexpr int XH = (KW + 1) / 2;  // x slots per half
+    constexpr int NT = 64 * k3Waves;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = wave_uniform(tid >> 6);
+    const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
+    const int c0 = blockIdx.y * k3Coils;
+    const int fr = (int)floorf(g.radius);
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    int org[3];
+    tile_origin3((int)(it.key % (uint32_t)g.ntiles_total), g, org);
+    const int sx = org[0] - fr, sy = org[1] - fr, sz = org[2] - fr;
+    const uint32_t ne = it.end - it.begin;  // 1..k3Seg
+    uint32_t* st = reinterpret_cast<uint32_t*>(sg + c2nc3_lds_floats(KW));
+    float* sd = reinterpret_cast<float*>(st + k3Seg * k3CDw);
+    const long long FM = (long long)F * M;
+
+    // the item's table: ne * 6 uint4, one per thread
+    constexpr int kU4 = k3CDw / 4;
+    static_assert(k3Seg * kU4 <= NT, "one table uint4 per thread");
+    const int nt4 = (int)ne * kU4;
+    uint4 ta = make_uint4(0u, 0u, 0u, 0u);
+    if (tid < nt4) ta = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * kU4 + tid];
+    float dj = 1.0f;
+    if (MODE == 1 && tid < (int)ne) dj = dcw[tab[((size_t)it.begin + tid) * k3CDw]];
+
+    // tile + halo: thread (cell group, coil, re/im) takes cells i, i + NT/16,
+    // ...; a wave's LDS stores cover 4 whole cells (64 consecutive floats).
+    // All of a thread's loads are issued before its first LDS store.
+    {
+        constexpr int kStep = NT / 16;
+        constexpr int kPer = (NCELL + kStep - 1) / kStep;
+        const int cc = tid & 15, i0 = tid >> 4;
+        const int coil = c0 + (cc >> 1), comp = cc & 1;
+        const bool cok = coil < C;
+        const float* gb = grid + 2 * L.sb * ((long long)(cok ? coil : c0) * F + f) + comp;
+        float v[kPer];
+        int par[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int cell = i0 + u * kStep;
+            const int ex = cell % E0, ey = (cell / E0) % E1, ez = cell / (E0 * E1);
+            int gx = sx + ex, gy = sy + ey, gz = sz + ez;
+            par[u] = (gx + gy + gz) & 1;  // the parity survives the wrap (even N_os)
+            gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
+            gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
+            gz = gz < 0 ? gz + g.nos[2] : (gz >= g.nos[2] ? gz - g.nos[2] : gz);
+            const bool ok = cok && cell < NCELL;
+            v[u] = ok ? gb[2 * ((long long)gx + L.sy * gy + L.sz * gz)] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int cell = i0 + u * kStep;
+            if (cell < NCELL) sg[cell * k3Cell + cc] = (sign && par[u]) ? -v[u] : v[u];
+        }
+        for (int i = tid; i < c2nc3_pad_cells(KW) * k3Cell; i += NT) sg[NCELL * k3Cell + i] = 0.0f;
+    }
+    if (tid < nt4) reinterpret_cast<uint4*>(st)[tid] = ta;
+    if (MODE == 1 && tid < (int)ne) sd[tid] = dj;
+    __syncthreads();
+
+    // lane = (slot, x half, coil, re/im); lanes past the last coil mirror coil
+    // c0 (same values, same output address)
+    const int slot = lane >> 5, xh = (lane >> 4) & 1, cc = lane & 15;
+    const int coil = c0 + (cc >> 1);
+    const bool on = coil < C;
+    const int lcoil = on ? coil : c0;
+    const int lcc = on ? cc : (cc & 1);
+    float* ol = out + 2 * (long long)lcoil * FM + (cc & 1);
+    const float* sl = sg + xh * XH * k3Cell + lcc;
+    // wave w takes a contiguous share of the entries, two at a time
+    const uint32_t q0 = (ne * (uint32_t)wave) / k3Waves, q1 = (ne * (uint32_t)(wave + 1)) / k3Waves;
+    for (uint32_t k = q0; k < q1; k += 2) {
+        const uint32_t e = min(k + (uint32_t)slot, q1 - 1);  // an odd tail repeats the last entry
+        const uint32_t* te = st + e * k3CDw;
+        const uint32_t j = te[0];
+        const uint32_t pk = te[1];
+        const int bx = (int)(pk & 0xff), by = (int)((pk >> 8) & 0xff), bz = (int)((pk >> 16) & 0xff);
+        float wx[XH], wy[KW], wz[KW];
+#pragma unroll
+        for (int i = 0; i < XH; i++) wx[i] = __uint_as_float(te[2 + xh * XH + i]);
+#pragma unroll
+        for (int i = 0; i < KW; i++) {
+            wy[i] = __uint_as_float(te[10 + i]);
+            wz[i] = __uint_as_float(te[17 + i]);
+        }
+        const float* base = sl + ((bz * E1 + by) * E0 + bx) * k3Cell;
+        float a = 0.0f;
+#pragma unroll
+        for (int iz = 0; iz < KW; iz++) {
+            float v[KW][XH];
+#pragma unroll
+            for (int iy = 0; iy < KW; iy++)
+#pragma unroll
+                for (int ix = 0; ix < XH; ix++) v[iy][ix] = base[((iz * E1 + iy) * E0 + ix) * k3Cell];
+            float pz = 0.0f;
+#pragma unroll
+            for (int iy = 0; iy < KW; iy++) {
+                float r = 0.0f;
+#pragma unroll
+                for (int ix = 0; ix < XH; ix++) r += wx[ix] * v[iy][ix];
+                pz += wy[iy] * r;
+            }
+            a += wz[iz] * pz;
+        }
+        // add the other x half (lane ^ 16)
+        a += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(a), 0x401F));
+        if (xh == 0) {
+            if constexpr (MODE == 1) a *= sd[e];
+            if constexpr (MODE == 2) {
+                const int dm = dcw_mode(dcw, dcw_len, FM);
+                if (dm == 1) a *= dcw[j];
+                else if (dm == 2) a *= dcw[dcw_index(lcoil, j, FM, dcw_len)];
+            }
+            const long long o = 2 * (long long)j;
+            if (MODE == 2 && accumulate) a += ol[o];
+            ol[o] = a;
+        }
+    }
+}
+
+// Home tile key of every sample (C2NC binning), any D.
+__global__ void home_keys_nd_kernel(const float* __restrict__ p, long long nsamp, long long M, GridGeom g,
+                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nsamp) return;
+    uint32_t t = 0, mul = 1;
+    for (int d = 0; d < g.D; d++) {
+        float pw;
+        t += mul * (uint32_t)home_tile_1d(p[j * g.D + d], g, d, pw);
+        mul *= (uint32_t)g.ntiles[d];
+    }
+    keys[j] = (uint32_t)(j / M) * (uint32_t)g.ntiles_total + t;
+    vals[j] = (uint32_t)j;
+}
+
+}  // namespace gt

@@ -40,6 +40,7 @@
 #include "gt_kernels.h"
 #include "gt_kernels_fast.h"
 #include "gt_kernels_reg.h"
+#include "gt_kernels_3d.h"
 #include "gt_fft.h"
 
 std::string& gt_host::last_error() {
@@ -144,6 +145,7 @@ struct gt_nufft_plan {
     int max_items_req = 0;
     bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
     bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
+    bool fast3d = false;     // the 3D hot path (register-tile NC2C, staged C2NC)
 
     gt::GridGeom geom{};  // geometry for the generic kernels
     std::vector<float> filt_host[3];
@@ -534,13 +536,56 @@ struct LaunchFastC2NC {
     }
 };
 
+struct LaunchTablesNC2C3 {
+    static void run(const float* pos, const uint32_t* list, const uint32_t* keys, long long n,
+                    const gt::GridGeom& g, float* wt, uint32_t* hd, hipStream_t s) {
+        hipLaunchKernelGGL(gt::nc2c_table3d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos, list,
+                           keys, n, g, wt, hd);
+    }
+};
+
+template <int KW>
+struct LaunchTablesC2NC3 {
+    static void run(const float* pos, const uint32_t* list, long long n, const gt::GridGeom& g, uint32_t* tab,
+                    hipStream_t s) {
+        hipLaunchKernelGGL((gt::c2nc_table3d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           pos, list, n, g, tab);
+    }
+};
+
+void launch_fast3_nc2c(const float* samples, float* grid, float* partials, const gt_nufft_plan* p, int C,
+                       const gt::Layout& L, int sign, const float* dcw, long long dcw_len, int accumulate,
+                       hipStream_t s) {
+    const unsigned chunks = (unsigned)((C + gt::k3Coils - 1) / gt::k3Coils);
+    const long long FM = (long long)(p->M * p->F);
+    auto k = !dcw ? gt::nc2c_reg3d_kernel<0> : (dcw_len == FM ? gt::nc2c_reg3d_kernel<1> : gt::nc2c_reg3d_kernel<2>);
+    hipLaunchKernelGGL(k, dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s, samples, grid, partials,
+                       p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(), p->bn.items.as<gt::WorkItem>(),
+                       int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw, dcw_len, accumulate);
+}
+
+template <int KW>
+struct LaunchFast3C2NC {
+    static void run(const float* grid, float* out, const gt_nufft_plan* p, int C, const gt::Layout& L, int sign,
+                    const float* dcw, long long dcw_len, int accumulate, hipStream_t s) {
+        const unsigned chunks = (unsigned)((C + gt::k3Coils - 1) / gt::k3Coils);
+        const size_t lds = size_t(gt::c2nc3_lds_bytes(KW));
+        const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
+        auto k = mode == 0 ? gt::c2nc_multi3d_kernel<KW, 0>
+                           : (mode == 1 ? gt::c2nc_multi3d_kernel<KW, 1> : gt::c2nc_multi3d_kernel<KW, 2>);
+        hipLaunchKernelGGL(k, dim3((unsigned)p->bc.n_items, chunks), dim3(64 * gt::k3Waves), lds, s, grid, out,
+                           p->bc.table.as<uint32_t>(), p->bc.items.as<gt::WorkItem>(), int(p->M), int(p->F), C,
+                           p->bc.geom, L, sign, dcw, dcw_len, accumulate);
+    }
+};
+
 int coil_chunk(const gt_nufft_plan* p) {
     // LDS budget per one-wave workgroup: 16 KiB of re/im accumulators
     int cc = int(16384 / (8 * std::max(1, p->geom.tile_cells)));
     return std::max(1, std::min(cc, 32));
 }
 
-gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile) {
+gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile, bool tile_covers_window = true) {
     gt::GridGeom g{};
     g.D = p->D;
     g.radius = p->radius;
@@ -553,8 +598,9 @@ gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile) {
             g.nos[d] = int(p->Nos[d]);
             g.nos_f[d] = float(unsigned(p->Nos[d]));
             g.beta[d] = p->beta[d];
-            // a window of KW cells must touch at most two tiles per dimension
-            const int t = std::min<int>(std::max(tile[d], p->KW), int(p->Nos[d]));
+            // the generic kernels: a window of KW cells touches at most two tiles
+            // per dimension; the 3D hot path's tables take any tile
+            const int t = std::min<int>(tile_covers_window ? std::max(tile[d], p->KW) : tile[d], int(p->Nos[d]));
             g.tile[d] = t;
             g.ntiles[d] = int((p->Nos[d] + t - 1) / t);
         } else {
@@ -598,6 +644,24 @@ void setup_geometry(gt_nufft_plan* p) {
             p->fast2d = true;
             p->bn.geom = make_geom(p, ft);
             p->bc.geom = make_geom(p, ft);
+        }
+    }
+    // 3D hot path: STANDARD gridding, a window of 3..7 cells, no requested
+    // tile, and a grid of at least tile + window cells per dimension.
+    p->fast3d = false;
+    if (p->D == 3 && p->conv_type == GT_CONV_STANDARD && p->KW >= 3 && p->KW <= gt::kTabKW &&
+        p->tile_req[0] <= 0 && p->tile_req[1] <= 0 && p->tile_req[2] <= 0) {
+        const int tn[3] = {gt::k3T0, gt::k3T1, gt::k3T2}, tc[3] = {gt::k3S0, gt::k3S1, gt::k3S2};
+        bool ok = true;
+        for (int d = 0; d < 3; d++) ok &= int(p->Nos[d]) >= std::max(tn[d], tc[d]) + p->KW;
+        const double fr = double(p->radius) - std::floor(double(p->radius));
+        const double eps = 4.0 * std::ldexp(1.0, -23) * double(std::max({p->Nos[0], p->Nos[1], p->Nos[2]}));
+        ok &= fr == 0.0 || (fr > eps && fr < 1.0 - eps);
+        ok &= gt::c2nc3_lds_bytes(p->KW) <= 65536;
+        if (ok) {
+            p->fast3d = true;
+            p->bn.geom = make_geom(p, tn, false);
+            p->bc.geom = make_geom(p, tc, false);
         }
     }
 }
@@ -676,8 +740,12 @@ std::vector<uint32_t> bin_home(gt_nufft_plan* p, const gt::GridGeom& g, Binning&
     DevBuf keys, vals, keys_sorted;
     keys.reserve(nsamp * 4);
     vals.reserve(nsamp * 4);
-    hipLaunchKernelGGL(gt::home_keys_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
-                       p->traj.as<float>(), nsamp, (long long)p->M, g, keys.as<uint32_t>(), vals.as<uint32_t>());
+    if (p->D == 2)
+        hipLaunchKernelGGL(gt::home_keys_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
+                           p->traj.as<float>(), nsamp, (long long)p->M, g, keys.as<uint32_t>(), vals.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(gt::home_keys_nd_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
+                           p->traj.as<float>(), nsamp, (long long)p->M, g, keys.as<uint32_t>(), vals.as<uint32_t>());
     GT_HIP(hipGetLastError());
     auto cnt = sort_pairs(keys, vals, size_t(nsamp), nkeys, keys_sorted, b.list, s);
     b.n = size_t(nsamp);
@@ -783,13 +851,14 @@ void make_items(gt_nufft_plan* p, Binning& b, const std::vector<uint32_t>& count
 // partial tile write and read, splitting a C2NC tile one more halo staging.
 size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& counts, size_t npairs,
                       bool nc2c) {
-    if (p->fast2d && !nc2c) {
-        // the 2D C2NC kernel stages a work item's whole table in LDS
-        const size_t cap = size_t(gt::kC2NCSeg);
+    if ((p->fast2d || p->fast3d) && !nc2c) {
+        // the C2NC kernels stage a work item's whole table in LDS
+        const size_t cap = size_t(p->fast2d ? gt::kC2NCSeg : gt::k3Seg);
         return p->max_items_req > 0 ? std::min(cap, size_t(p->max_items_req)) : cap;
     }
     if (p->max_items_req > 0) return size_t(p->max_items_req);
     if (p->fast2d) return 256;
+    if (p->fast3d) return 1024;
     const double avg = double(npairs) / double(std::max<size_t>(counts.size(), 1));
     return size_t(std::max(256.0, 4.0 * avg));
 }
@@ -808,6 +877,17 @@ void build_nc2c(gt_nufft_plan* p, hipStream_t s) {
         GT_HIP(hipGetLastError());
         make_items(p, b, counts, segment_length(p, counts, b.n, true), true, s);
         keys_sorted.release();
+    } else if (p->fast3d) {
+        DevBuf keys_sorted;
+        auto counts = bin_windows(p, b.geom, b, &keys_sorted, s);
+        b.table.reserve(std::max<size_t>(b.n, 1) * gt::k3NW * sizeof(float));
+        b.heads.reserve(std::max<size_t>(b.n, 1) * 2 * sizeof(uint32_t));
+        if (b.n)
+            LaunchTablesNC2C3::run(p->traj.as<float>(), b.list.as<uint32_t>(), keys_sorted.as<uint32_t>(),
+                                   (long long)b.n, b.geom, b.table.as<float>(), b.heads.as<uint32_t>(), s);
+        GT_HIP(hipGetLastError());
+        make_items(p, b, counts, segment_length(p, counts, b.n, true), true, s);
+        keys_sorted.release();
     } else {
         b.geom = p->geom;
         auto counts = bin_windows(p, b.geom, b, nullptr, s);
@@ -816,12 +896,18 @@ void build_nc2c(gt_nufft_plan* p, hipStream_t s) {
 }
 
 void build_c2nc(gt_nufft_plan* p, hipStream_t s) {
-    if (!p->fast2d) return;  // the generic C2NC gathers per sample without binning
+    if (!p->fast2d && !p->fast3d) return;  // the generic C2NC gathers per sample without binning
     Binning& b = p->bc;
     auto counts = bin_home(p, b.geom, b, s);
-    b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
-    dispatch_fast<LaunchTablesC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n, b.geom,
-                                    b.table.as<uint32_t>(), s);
+    if (p->fast2d) {
+        b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
+        dispatch_fast<LaunchTablesC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
+                                        b.geom, b.table.as<uint32_t>(), s);
+    } else {
+        b.table.reserve(std::max<size_t>(b.n, 1) * gt::k3CDw * 4);
+        dispatch_fast<LaunchTablesC2NC3>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
+                                         b.geom, b.table.as<uint32_t>(), s);
+    }
     GT_HIP(hipGetLastError());
     make_items(p, b, counts, segment_length(p, counts, b.n, false), false, s);
 }
@@ -875,6 +961,9 @@ void conv_c2nc(gt_nufft_plan* p, const void* grid, const gt::Layout& L, void* ou
     if (p->fast2d) {
         dispatch_fast<LaunchFastC2NC>(p->KW, (const float*)grid, (float*)out, p, int(C), L, sign, dcw, dl,
                                       accumulate ? 1 : 0, s);
+    } else if (p->fast3d) {
+        dispatch_fast<LaunchFast3C2NC>(p->KW, (const float*)grid, (float*)out, p, int(C), L, sign, dcw, dl,
+                                       accumulate ? 1 : 0, s);
     } else {
         dispatch<LaunchC2NC>(p->D, p->KW, (const float2*)grid, (float2*)out, p->traj.as<float>(), int(p->M),
                              int(p->F), int(C), p->geom, sign, dcw, dl, accumulate ? 1 : 0, s);
@@ -904,6 +993,19 @@ void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layo
         if (b.n_red > 0) {
             const unsigned parts = (unsigned)((C * gt::kRT * gt::kRT * 2 + 255) / 256);
             hipLaunchKernelGGL(gt::nc2c_reduce2d_kernel, dim3((unsigned)b.n_red, parts), dim3(256), 0, s,
+                               p->partials.as<float>(), (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                               b.geom, L, sign, accumulate ? 1 : 0);
+            GT_HIP(hipGetLastError());
+        }
+        return;
+    }
+    if (p->fast3d) {
+        launch_fast3_nc2c((const float*)samples, (float*)grid, p->partials.as<float>(), p, int(C), L, sign, dcw,
+                          dl, accumulate ? 1 : 0, s);
+        GT_HIP(hipGetLastError());
+        if (b.n_red > 0) {
+            const unsigned parts = (unsigned)((C * b.geom.tile_cells * 2 + 255) / 256);
+            hipLaunchKernelGGL(gt::nc2c_reduce3d_kernel, dim3((unsigned)b.n_red, parts), dim3(256), 0, s,
                                p->partials.as<float>(), (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
                                b.geom, L, sign, accumulate ? 1 : 0);
             GT_HIP(hipGetLastError());
