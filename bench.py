@@ -88,13 +88,21 @@ def cpu_baseline(wl, samples, budget_s):
 
 
 KERNELS_2D = {"nc2c": "nc2c_reg2d_kernel + nc2c_reduce2d_kernel", "c2nc": "c2nc_multi2d_kernel"}
+KERNELS_3D = {"nc2c": "nc2c_reg3d_kernel + nc2c_reduce3d_kernel", "c2nc": "c2nc_multi3d_kernel"}
 KERNELS_GENERIC = {"nc2c": "nc2c_tile_kernel + nc2c_reduce_kernel", "c2nc": "c2nc_gather_kernel"}
+FP32_PEAK_TFLOPS = 157.3
 
 
-def plan_is_fast2d(N):
-    # the library takes its 2D hot path for every workload here (2D, W 5.5,
-    # 8 x 8 tiles); 1D and 3D use the generic kernels
-    return len(N) == 2
+def gridding_kernels(N):
+    # the library takes its 2D and 3D hot paths for every workload here
+    # (W 5.5, default tiles); 1D uses the generic kernels
+    return {2: KERNELS_2D, 3: KERNELS_3D}.get(len(N), KERNELS_GENERIC)
+
+
+def gridding_flops(N, M, C, W=5.5):
+    """Algorithmic FP32 flops of one gridding direction: W^D cells per sample
+    and coil, a complex-times-real multiply-add (4 flops) each."""
+    return 4.0 * W ** len(N) * M * C
 
 
 def copy_probe(mib, reps):
@@ -299,7 +307,8 @@ def main():
     B, G = grid_bytes(N, M, C, D)
     nc2c_ms = stages["nc2c"][0] / max(stages["nc2c"][1], 1)
     c2nc_ms = stages["c2nc"][0] / max(stages["c2nc"][1], 1)
-    kernels = KERNELS_2D if plan_is_fast2d(N) else KERNELS_GENERIC
+    kernels = gridding_kernels(N)
+    flops = gridding_flops(N, M, C)
     # the dominant gridding stage is the roofline's kernel; both are reported
     dom = "nc2c" if nc2c_ms >= c2nc_ms else "c2nc"
     dom_ms = max(nc2c_ms, c2nc_ms)
@@ -318,6 +327,11 @@ def main():
         "stage_frac": {"nc2c": round(B / (nc2c_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "c2nc": round(B / (c2nc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "stage_kernels": kernels,
+        # the FP32 vector side of the same stages (the 3D window makes gridding
+        # compute-heavy: 665.5 flop per sample and coil against 121 in 2D)
+        "stage_valu_tflops": {"nc2c": round(flops / (nc2c_ms * 1e-3) / 1e12, 2),
+                              "c2nc": round(flops / (c2nc_ms * 1e-3) / 1e12, 2),
+                              "peak": FP32_PEAK_TFLOPS},
     }
 
     # parity of this run's output on one coil against the oracle is covered by

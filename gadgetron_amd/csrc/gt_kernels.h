@@ -157,10 +157,22 @@ __global__ void emit_pairs_kernel(const float* __restrict__ p, long long nsamp, 
             }
 }
 
-__global__ void histogram_keys_kernel(const uint32_t* __restrict__ keys, long long n,
-                                      uint32_t* __restrict__ hist) {
+// Entry ranges of a sorted key array: first[k] / last[k] bound the run of key k
+// (both stay 0 for an absent key).  Found from the run boundaries, since
+// atomic counting of sorted keys would serialise on every run.
+__global__ void run_bounds_kernel(const uint32_t* __restrict__ keys, long long n, uint32_t* __restrict__ first,
+                                  uint32_t* __restrict__ last) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&hist[keys[i]], 1u);
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) first[k] = (uint32_t)i;
+    if (i == n - 1 || keys[i + 1] != k) last[k] = (uint32_t)(i + 1);
+}
+
+__global__ void run_counts_kernel(const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
+                                  uint32_t nkeys, uint32_t* __restrict__ counts) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nkeys) counts[k] = last[k] - first[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -577,91 +589,77 @@ struct PadCrop {
     int in_os;
 };
 
-__global__ __launch_bounds__(256) void pad_deapod_kernel(const float2* __restrict__ img,
-                                                         float2* __restrict__ grid, long long nb,
-                                                         GridGeom g, PadCrop pc) {
-    const long long G = g.G;
-    const long long total = nb * G;
-    long long n = 1;
-    for (int d = 0; d < g.D; d++) n *= pc.N[d];
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const long long b = i / G;
-        long long r = i - b * G;
-        int gc[3] = {0, 0, 0};
-        for (int d = 0; d < g.D; d++) {
-            gc[d] = (int)(r % g.nos[d]);
-            r /= g.nos[d];
-        }
+// Pad (or copy) + deapodize + centring sign + scale, caller layout, D <= 3.
+// A 64-thread row per grid line R = ((b * Oz + z) * Oy + y): the row's band
+// test, filter product and sign are worked out once, the threads run along x.
+constexpr int kRowsPerBlock = 4;
+
+__global__ __launch_bounds__(64 * kRowsPerBlock) void pad_deapod_kernel(const float2* __restrict__ img,
+                                                                        float2* __restrict__ grid, long long nb,
+                                                                        GridGeom g, PadCrop pc) {
+    const long long R = (long long)blockIdx.x * kRowsPerBlock + threadIdx.y;
+    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2];
+    if (R >= nb * Oy * Oz) return;
+    const int y = (int)(R % Oy);
+    const long long t = R / Oy;
+    const int z = (int)(t % Oz);
+    const long long b = t / Oz;
+    float2* out = grid + R * Ox;
+    const int iy = pc.in_os ? y : y - pc.off[1], iz = pc.in_os ? z : z - pc.off[2];
+    const int Nx = pc.in_os ? Ox : pc.N[0], Ny = pc.in_os ? Oy : pc.N[1], Nz = pc.in_os ? Oz : pc.N[2];
+    const int offx = pc.in_os ? 0 : pc.off[0];
+    if (iy < 0 || iy >= Ny || iz < 0 || iz >= Nz) {
+        for (int x = threadIdx.x; x < Ox; x += 64) out[x] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const float2* in = img + ((b * Nz + iz) * Ny + iy) * Nx;
+    float sr = pc.scale;
+    if (g.D >= 2) sr *= pc.filt[1][y];
+    if (g.D >= 3) sr *= pc.filt[2][z];
+    if (pc.sign && ((y + z) & 1)) sr = -sr;
+    for (int x = threadIdx.x; x < Ox; x += 64) {
+        const int ix = x - offx;
         float2 v = make_float2(0.0f, 0.0f);
-        bool inside = true;
-        long long src = 0;
-        if (pc.in_os) {
-            src = b * G + (i - b * G);
-        } else {
-            long long st = 1;
-            for (int d = 0; d < g.D; d++) {
-                const int c = gc[d] - pc.off[d];
-                inside &= (c >= 0 && c < pc.N[d]);
-                src += (long long)c * st;
-                st *= pc.N[d];
-            }
-            src += b * n;
-        }
-        if (inside) {
-            v = img[src];
-            float s = pc.scale;
-            int par = 0;
-            for (int d = 0; d < g.D; d++) {
-                s *= pc.filt[d][gc[d]];
-                par += gc[d];
-            }
-            if (pc.sign && (par & 1)) s = -s;
+        if (ix >= 0 && ix < Nx) {
+            v = in[ix];
+            const float s = sr * pc.filt[0][x] * ((pc.sign && (x & 1)) ? -1.0f : 1.0f);
             v.x *= s;
             v.y *= s;
         }
-        grid[i] = v;
+        out[x] = v;
     }
 }
 
-// Crop (offset (N_os - N) >> 1, NFFT.hpp:178) + deapodize + centring sign + scale.
-// With out_os the whole grid is processed in place (in may equal out).
-__global__ __launch_bounds__(256) void crop_deapod_kernel(const float2* in,
-                                                          float2* out, long long nb,
-                                                          GridGeom g, PadCrop pc) {
-    const long long G = g.G;
-    long long n = 1;
-    for (int d = 0; d < g.D; d++) n *= (pc.in_os ? g.nos[d] : pc.N[d]);
-    const long long total = nb * n;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const long long b = i / n;
-        long long r = i - b * n;
-        int gc[3] = {0, 0, 0};
-        long long src = 0, st = 1;
-        for (int d = 0; d < g.D; d++) {
-            const int ext = pc.in_os ? g.nos[d] : pc.N[d];
-            const int c = (int)(r % ext);
-            r /= ext;
-            gc[d] = pc.in_os ? c : c + pc.off[d];
-            src += (long long)gc[d] * st;
-            st *= g.nos[d];
-        }
-        float2 v = in[b * G + src];
-        float s = pc.scale;
-        int par = 0;
-        for (int d = 0; d < g.D; d++) {
-            s *= pc.filt[d][gc[d]];
-            par += gc[d];
-        }
-        if (pc.sign && (par & 1)) s = -s;
+// Crop (or copy) + deapodize + centring sign + scale, caller layout, D <= 3:
+// a 64-thread row per output line.
+__global__ __launch_bounds__(64 * kRowsPerBlock) void crop_deapod_kernel(const float2* in, float2* out,
+                                                                         long long nb, GridGeom g, PadCrop pc) {
+    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2];
+    const int Nx = pc.in_os ? Ox : pc.N[0], Ny = pc.in_os ? Oy : pc.N[1], Nz = pc.in_os ? Oz : pc.N[2];
+    const long long R = (long long)blockIdx.x * kRowsPerBlock + threadIdx.y;
+    if (R >= nb * Ny * Nz) return;
+    const int iy = (int)(R % Ny);
+    const long long t = R / Ny;
+    const int iz = (int)(t % Nz);
+    const long long b = t / Nz;
+    const int y = pc.in_os ? iy : iy + pc.off[1], z = pc.in_os ? iz : iz + pc.off[2];
+    const int offx = pc.in_os ? 0 : pc.off[0];
+    const float2* src = in + ((b * Oz + z) * Oy + y) * (long long)Ox + offx;
+    float2* dst = out + R * Nx;
+    float sr = pc.scale;
+    if (g.D >= 2) sr *= pc.filt[1][y];
+    if (g.D >= 3) sr *= pc.filt[2][z];
+    if (pc.sign && ((y + z) & 1)) sr = -sr;
+    for (int ix = threadIdx.x; ix < Nx; ix += 64) {
+        const int x = ix + offx;
+        float2 v = src[ix];
+        const float s = sr * pc.filt[0][x] * ((pc.sign && (x & 1)) ? -1.0f : 1.0f);
         v.x *= s;
         v.y *= s;
-        out[i] = v;
+        dst[ix] = v;
     }
 }
 
-// Multiply an oversampled array by the centring sign (-1)^(x+y+z) times a scale.
 __global__ __launch_bounds__(256) void sign_scale_kernel(float2* __restrict__ data, long long nb,
                                                          GridGeom g, float scale, int sign) {
     const long long G = g.G;

@@ -11,13 +11,12 @@
 //   each lane sums its cells in table order (bitwise repeatable).
 //
 // c2nc_multi3d_kernel (NFFT_convolve_kernel, ConvolverC2NC.cuh:260-307)
-//   A workgroup stages one 4 x 4 x 4 tile plus the kernel halo for 8 coils in
-//   LDS, cell-major with the 16 (coil, re/im) floats of a cell contiguous.
-//   Each wavefront then gathers two samples at a time: lane (sample slot,
-//   x half, coil, re/im) sums its half of every x row of the window, and the
-//   two halves are added across lanes.  A read instruction covers two runs
-//   of 16 consecutive floats per sample, so the window reads are free of
-//   bank conflicts.
+//   A persistent workgroup stages one 4 x 4 x 4 tile plus the kernel halo
+//   for 8 coils in LDS, one plane per coil in the grid's own order, so the
+//   staging loads read whole grid rows; the next item's staging is in flight
+//   while the current one is gathered.  Each wavefront gathers two samples at
+//   a time: lane (sample slot, x half, coil, re/im) sums its half of every x
+//   row of the window, and the two halves are added across lanes.
 #pragma once
 
 #include "gt_kernels_reg.h"
@@ -330,86 +329,149 @@ __global__ void c2nc_table3d_kernel(const float* __restrict__ p, const uint32_t*
     t[1] = packed;
 }
 
-// Staged cells: E0 x E1 x E2 (tile + KW - 1 per dimension), then zeroed pad
-// cells: a window reads up to one cell, row and plane past the staged tile
-// (and, with the x halves, 2 XH - KW more cells), always with weight 0.
+// LDS of the C2NC kernel: one plane per coil holding the staged cells
+// E0 x E1 x E2 (tile + KW - 1 per dimension) as (re, im) pairs in the grid's
+// own order (x fastest), then zeroed pad cells -- a window reads up to one cell,
+// row and plane past the staged tile (and, with the x halves, 2 XH - KW more
+// cells), always with weight 0.  The plane stride is 4 (mod 32) floats: the 16
+// (coil, re/im) lanes of one x half then fall on banks {4c + comp}, and the
+// other half, 2 XH floats on (XH = 3 for W 5.5), on the remaining 16 banks.
 __host__ __device__ constexpr int c2nc3_cells(int KW) { return (k3S0 + KW - 1) * (k3S1 + KW - 1) * (k3S2 + KW - 1); }
 __host__ __device__ constexpr int c2nc3_pad_cells(int KW) {
     return (k3S0 + KW - 1) * (k3S1 + KW - 1) + (k3S0 + KW - 1) + 8;
 }
-__host__ __device__ constexpr int c2nc3_lds_floats(int KW) { return (c2nc3_cells(KW) + c2nc3_pad_cells(KW)) * k3Cell; }
+__host__ __device__ constexpr int c2nc3_plane_floats(int KW) {
+    return ((2 * (c2nc3_cells(KW) + c2nc3_pad_cells(KW)) + 31) / 32) * 32 + 4;
+}
+__host__ __device__ constexpr int c2nc3_lds_floats(int KW) { return k3Coils * c2nc3_plane_floats(KW); }
 __host__ __device__ constexpr int c2nc3_lds_bytes(int KW) { return (c2nc3_lds_floats(KW) + k3Seg * k3CDw + k3Seg) * 4; }
 
-// MODE 0: no dcw, no accumulation; 1: one dcw weight per sample (staged with
-// the table), no accumulation; 2: anything else.
+// Staging of one work item (tile + halo for 8 coils, and its table) into
+// registers.  Wave w stages coil c0 + w; its lanes (row r, float f) cover R
+// grid rows of 2 E0 floats at a time, so a load instruction reads R runs of
+// 2 E0 consecutive floats and the LDS stores are contiguous.  Row and plane
+// wraps and centring signs are worked out once per row and plane.
+template <int KW>
+struct C2NC3Stage {
+    static constexpr int E0 = k3S0 + KW - 1, E1 = k3S1 + KW - 1, E2 = k3S2 + KW - 1;
+    static constexpr int NT = 64 * k3Waves;
+    static constexpr int RF = 2 * E0;               // floats per staged row
+    static constexpr int R = kLanes / RF;           // rows per load instruction
+    static constexpr int kYI = (E1 + R - 1) / R;    // load instructions per plane
+    static constexpr int kU4 = k3CDw / 4;           // table uint4 per entry
+    static_assert(k3Seg * kU4 <= NT, "one table uint4 per thread");
+    static_assert(k3Waves == k3Coils, "one wave per coil");
+    float v[E2][kYI];
+    int par;  // bit m * E2 + ez: centring sign of value (ez, m)
+    uint4 ta;
+    float dj;  // dcw of entry tid (per-entry dcw array, or 1)
+    uint32_t key, ne;
+    bool noop;
+
+    __device__ __forceinline__ void load(const float* __restrict__ grid, const uint32_t* __restrict__ tab,
+                                         const float* __restrict__ dcw_e, const WorkItem& it, int tid, int c0,
+                                         int C, int F, const GridGeom& g, const Layout& L, int sign) {
+        noop = it.partial == kNoopItem;
+        key = it.key;
+        ne = noop ? 0u : it.end - it.begin;
+        const int nt4 = (int)ne * kU4;
+        ta = make_uint4(0u, 0u, 0u, 0u);
+        if (tid < nt4) ta = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * kU4 + tid];
+        dj = 1.0f;
+        if (dcw_e && tid < (int)ne) dj = dcw_e[it.begin + tid];
+        const int fr = (int)floorf(g.radius);
+        const int f = (int)(key / (uint32_t)g.ntiles_total);
+        int org[3];
+        tile_origin3((int)(key % (uint32_t)g.ntiles_total), g, org);
+        const int sx = org[0] - fr, sy = org[1] - fr, sz = org[2] - fr;
+        const int lane = tid & 63, coil = c0 + (tid >> 6);
+        const int r = lane / RF, fl = lane % RF;  // row in the instruction, float in the row
+        const int ex = fl >> 1, comp = fl & 1;
+        const bool cok = !noop && r < R && coil < C;
+        int gx = sx + ex;
+        const int px = gx & 1;
+        gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
+        const float* gb = grid + 2 * ((long long)gx + L.sb * ((long long)(cok ? coil : c0) * F + f)) + comp;
+        long long oy[kYI];
+        int py[kYI];
+#pragma unroll
+        for (int m = 0; m < kYI; m++) {
+            int gy = sy + r + m * R;
+            py[m] = gy & 1;
+            gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
+            oy[m] = 2 * L.sy * gy;
+        }
+        par = 0;
+#pragma unroll
+        for (int ez = 0; ez < E2; ez++) {
+            int gz = sz + ez;
+            // the parity of a wrapped coordinate equals that of the unwrapped one (even N_os)
+            const int pz = gz & 1;
+            gz = gz < 0 ? gz + g.nos[2] : (gz >= g.nos[2] ? gz - g.nos[2] : gz);
+            const float* pl = gb + 2 * L.sz * gz;
+#pragma unroll
+            for (int m = 0; m < kYI; m++) {
+                const bool ok = cok && r + m * R < E1;
+                v[ez][m] = ok ? pl[oy[m]] : 0.0f;
+                par |= (sign && ((px ^ py[m] ^ pz) & 1)) ? (1 << (m * E2 + ez)) : 0;
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(float* sg, uint32_t* st, float* sd, int tid) const {
+        const int lane = tid & 63;
+        const int r = lane / RF, fl = lane % RF;
+        if (r < R) {
+            float* d = sg + (tid >> 6) * c2nc3_plane_floats(KW) + r * RF + fl;
+#pragma unroll
+            for (int ez = 0; ez < E2; ez++)
+#pragma unroll
+                for (int m = 0; m < kYI; m++) {
+                    if (r + m * R >= E1) continue;
+                    const float x = v[ez][m];
+                    d[(ez * E1 + m * R) * RF] = ((par >> (m * E2 + ez)) & 1) ? -x : x;
+                }
+        }
+        if (tid < (int)ne * kU4) reinterpret_cast<uint4*>(st)[tid] = ta;
+        if (tid < (int)ne) sd[tid] = dj;
+    }
+};
+
+// dcw of every C2NC table entry (the samples' dcw in table order), so that the
+// gridding kernel loads it with the table instead of through a dependent load.
+__global__ void entry_dcw_kernel(const uint32_t* __restrict__ list, long long n, const float* __restrict__ dcw,
+                                 float* __restrict__ out) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = dcw[list[e]];
+}
+
+// MODE 0: no dcw, no accumulation; 1: one dcw weight per sample (dcw_e: per
+// table entry), no accumulation; 2: anything else.
+// Persistent: workgroup b takes items b, b + gridDim.x, ... (gridDim.x a
+// multiple of 8 keeps each item on its XCD).  While it gathers one item, the
+// next item's staging loads are in flight in registers.
 template <int KW, int MODE>
 __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
     const float* __restrict__ grid, float* __restrict__ out, const uint32_t* __restrict__ tab,
-    const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g, Layout L, int sign,
-    const float* __restrict__ dcw, long long dcw_len, int accumulate) {
-    extern __shared__ float sg[];  // staged grid, then the item's table, then its dcw
-    constexpr int E0 = k3S0 + KW - 1, E1 = k3S1 + KW - 1;
+    const WorkItem* __restrict__ items, int n_items, int M, int F, int C, GridGeom g, Layout L, int sign,
+    const float* __restrict__ dcw, long long dcw_len, const float* __restrict__ dcw_e, int accumulate) {
+    extern __shared__ float sg[];  // coil planes, then the item's table, then its dcw
+    using S = C2NC3Stage<KW>;
+    constexpr int E0 = S::E0, E1 = S::E1;
     constexpr int NCELL = c2nc3_cells(KW);
+    constexpr int PS = c2nc3_plane_floats(KW);
     constexpr int XH = (KW + 1) / 2;  // x slots per half
-    constexpr int NT = 64 * k3Waves;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = wave_uniform(tid >> 6);
-    const WorkItem it = items[blockIdx.x];
-    if (it.partial == kNoopItem) return;
     const int c0 = blockIdx.y * k3Coils;
-    const int fr = (int)floorf(g.radius);
-    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
-    int org[3];
-    tile_origin3((int)(it.key % (uint32_t)g.ntiles_total), g, org);
-    const int sx = org[0] - fr, sy = org[1] - fr, sz = org[2] - fr;
-    const uint32_t ne = it.end - it.begin;  // 1..k3Seg
     uint32_t* st = reinterpret_cast<uint32_t*>(sg + c2nc3_lds_floats(KW));
     float* sd = reinterpret_cast<float*>(st + k3Seg * k3CDw);
     const long long FM = (long long)F * M;
 
-    // the item's table: ne * 6 uint4, one per thread
-    constexpr int kU4 = k3CDw / 4;
-    static_assert(k3Seg * kU4 <= NT, "one table uint4 per thread");
-    const int nt4 = (int)ne * kU4;
-    uint4 ta = make_uint4(0u, 0u, 0u, 0u);
-    if (tid < nt4) ta = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * kU4 + tid];
-    float dj = 1.0f;
-    if (MODE == 1 && tid < (int)ne) dj = dcw[tab[((size_t)it.begin + tid) * k3CDw]];
-
-    // tile + halo: thread (cell group, coil, re/im) takes cells i, i + NT/16,
-    // ...; a wave's LDS stores cover 4 whole cells (64 consecutive floats).
-    // All of a thread's loads are issued before its first LDS store.
-    {
-        constexpr int kStep = NT / 16;
-        constexpr int kPer = (NCELL + kStep - 1) / kStep;
-        const int cc = tid & 15, i0 = tid >> 4;
-        const int coil = c0 + (cc >> 1), comp = cc & 1;
-        const bool cok = coil < C;
-        const float* gb = grid + 2 * L.sb * ((long long)(cok ? coil : c0) * F + f) + comp;
-        float v[kPer];
-        int par[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            const int cell = i0 + u * kStep;
-            const int ex = cell % E0, ey = (cell / E0) % E1, ez = cell / (E0 * E1);
-            int gx = sx + ex, gy = sy + ey, gz = sz + ez;
-            par[u] = (gx + gy + gz) & 1;  // the parity survives the wrap (even N_os)
-            gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
-            gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
-            gz = gz < 0 ? gz + g.nos[2] : (gz >= g.nos[2] ? gz - g.nos[2] : gz);
-            const bool ok = cok && cell < NCELL;
-            v[u] = ok ? gb[2 * ((long long)gx + L.sy * gy + L.sz * gz)] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; u++) {
-            const int cell = i0 + u * kStep;
-            if (cell < NCELL) sg[cell * k3Cell + cc] = (sign && par[u]) ? -v[u] : v[u];
-        }
-        for (int i = tid; i < c2nc3_pad_cells(KW) * k3Cell; i += NT) sg[NCELL * k3Cell + i] = 0.0f;
+    for (int i = tid; i < k3Coils * (PS - 2 * NCELL); i += S::NT) {
+        const int c = i / (PS - 2 * NCELL), o = i % (PS - 2 * NCELL);
+        sg[c * PS + 2 * NCELL + o] = 0.0f;
     }
-    if (tid < nt4) reinterpret_cast<uint4*>(st)[tid] = ta;
-    if (MODE == 1 && tid < (int)ne) sd[tid] = dj;
-    __syncthreads();
 
     // lane = (slot, x half, coil, re/im); lanes past the last coil mirror coil
     // c0 (same values, same output address)
@@ -419,54 +481,70 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
     const int lcoil = on ? coil : c0;
     const int lcc = on ? cc : (cc & 1);
     float* ol = out + 2 * (long long)lcoil * FM + (cc & 1);
-    const float* sl = sg + xh * XH * k3Cell + lcc;
-    // wave w takes a contiguous share of the entries, two at a time
-    const uint32_t q0 = (ne * (uint32_t)wave) / k3Waves, q1 = (ne * (uint32_t)(wave + 1)) / k3Waves;
-    for (uint32_t k = q0; k < q1; k += 2) {
-        const uint32_t e = min(k + (uint32_t)slot, q1 - 1);  // an odd tail repeats the last entry
-        const uint32_t* te = st + e * k3CDw;
-        const uint32_t j = te[0];
-        const uint32_t pk = te[1];
-        const int bx = (int)(pk & 0xff), by = (int)((pk >> 8) & 0xff), bz = (int)((pk >> 16) & 0xff);
-        float wx[XH], wy[KW], wz[KW];
+    const float* sl = sg + (lcc >> 1) * PS + (lcc & 1) + 2 * xh * XH;
+
+    const float* de = MODE == 1 ? dcw_e : nullptr;
+    S cur;
+    int item = blockIdx.x;
+    if (item < n_items) cur.load(grid, tab, de, items[item], tid, c0, C, F, g, L, sign);
+    for (; item < n_items; item += gridDim.x) {
+        __syncthreads();  // the previous item's gather is done with LDS
+        cur.store(sg, st, sd, tid);
+        __syncthreads();
+        const uint32_t ne = cur.ne;
+        const bool noop = cur.noop;
+        // the next item's staging is in flight during this item's gather
+        const int nxt = item + (int)gridDim.x;
+        if (nxt < n_items) cur.load(grid, tab, de, items[nxt], tid, c0, C, F, g, L, sign);
+        if (noop) continue;
+        // wave w takes a contiguous share of the entries, two at a time
+        const uint32_t q0 = (ne * (uint32_t)wave) / k3Waves, q1 = (ne * (uint32_t)(wave + 1)) / k3Waves;
+        for (uint32_t k = q0; k < q1; k += 2) {
+            const uint32_t e = min(k + (uint32_t)slot, q1 - 1);  // an odd tail repeats the last entry
+            const uint32_t* te = st + e * k3CDw;
+            const uint32_t j = te[0];
+            const uint32_t pk = te[1];
+            const int bx = (int)(pk & 0xff), by = (int)((pk >> 8) & 0xff), bz = (int)((pk >> 16) & 0xff);
+            float wx[XH], wy[KW], wz[KW];
 #pragma unroll
-        for (int i = 0; i < XH; i++) wx[i] = __uint_as_float(te[2 + xh * XH + i]);
+            for (int i = 0; i < XH; i++) wx[i] = __uint_as_float(te[2 + xh * XH + i]);
 #pragma unroll
-        for (int i = 0; i < KW; i++) {
-            wy[i] = __uint_as_float(te[10 + i]);
-            wz[i] = __uint_as_float(te[17 + i]);
-        }
-        const float* base = sl + ((bz * E1 + by) * E0 + bx) * k3Cell;
-        float a = 0.0f;
-#pragma unroll
-        for (int iz = 0; iz < KW; iz++) {
-            float v[KW][XH];
-#pragma unroll
-            for (int iy = 0; iy < KW; iy++)
-#pragma unroll
-                for (int ix = 0; ix < XH; ix++) v[iy][ix] = base[((iz * E1 + iy) * E0 + ix) * k3Cell];
-            float pz = 0.0f;
-#pragma unroll
-            for (int iy = 0; iy < KW; iy++) {
-                float r = 0.0f;
-#pragma unroll
-                for (int ix = 0; ix < XH; ix++) r += wx[ix] * v[iy][ix];
-                pz += wy[iy] * r;
+            for (int i = 0; i < KW; i++) {
+                wy[i] = __uint_as_float(te[10 + i]);
+                wz[i] = __uint_as_float(te[17 + i]);
             }
-            a += wz[iz] * pz;
-        }
-        // add the other x half (lane ^ 16)
-        a += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(a), 0x401F));
-        if (xh == 0) {
-            if constexpr (MODE == 1) a *= sd[e];
-            if constexpr (MODE == 2) {
-                const int dm = dcw_mode(dcw, dcw_len, FM);
-                if (dm == 1) a *= dcw[j];
-                else if (dm == 2) a *= dcw[dcw_index(lcoil, j, FM, dcw_len)];
+            const float* base = sl + 2 * ((bz * E1 + by) * E0 + bx);
+            float a = 0.0f;
+#pragma unroll
+            for (int iz = 0; iz < KW; iz++) {
+                float v[KW][XH];
+#pragma unroll
+                for (int iy = 0; iy < KW; iy++)
+#pragma unroll
+                    for (int ix = 0; ix < XH; ix++) v[iy][ix] = base[2 * ((iz * E1 + iy) * E0 + ix)];
+                float pz = 0.0f;
+#pragma unroll
+                for (int iy = 0; iy < KW; iy++) {
+                    float r = 0.0f;
+#pragma unroll
+                    for (int ix = 0; ix < XH; ix++) r += wx[ix] * v[iy][ix];
+                    pz += wy[iy] * r;
+                }
+                a += wz[iz] * pz;
             }
-            const long long o = 2 * (long long)j;
-            if (MODE == 2 && accumulate) a += ol[o];
-            ol[o] = a;
+            // add the other x half (lane ^ 16)
+            a += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(a), 0x401F));
+            if (xh == 0) {
+                if constexpr (MODE == 1) a *= sd[e];
+                if constexpr (MODE == 2) {
+                    const int dm = dcw_mode(dcw, dcw_len, FM);
+                    if (dm == 1) a *= dcw[j];
+                    else if (dm == 2) a *= dcw[dcw_index(lcoil, j, FM, dcw_len)];
+                }
+                const long long o = 2 * (long long)j;
+                if (MODE == 2 && accumulate) a += ol[o];
+                ol[o] = a;
+            }
         }
     }
 }

@@ -116,12 +116,14 @@ struct Binning {
     DevBuf list;      // sample index per entry, sorted by tile key then sample
     DevBuf table;     // per-entry Kaiser-Bessel table (2D hot path)
     DevBuf heads;     // per-entry (sample index, row mask) of the NC2C table
+    DevBuf dcw_e;     // dcw per entry (3D C2NC, per-sample dcw)
     DevBuf items;     // gt::WorkItem
     DevBuf red;       // gt::RedItem (split tiles)
     size_t n = 0;
     int n_items = 0, n_red = 0, n_partials = 0;
     void release() {
         list.release();
+        dcw_e.release();
         table.release();
         heads.release();
         items.release();
@@ -140,6 +142,7 @@ struct gt_nufft_plan {
     float Wf = 0, radius = 0, one_over_W = 0, beta[3] = {0, 0, 0};
     int conv_type = GT_CONV_STANDARD;
     int device = 0;
+    int cus = 0;  // compute units of the device
     int KW = 0;
     int tile_req[3] = {0, 0, 0};
     int max_items_req = 0;
@@ -566,16 +569,26 @@ void launch_fast3_nc2c(const float* samples, float* grid, float* partials, const
 
 template <int KW>
 struct LaunchFast3C2NC {
-    static void run(const float* grid, float* out, const gt_nufft_plan* p, int C, const gt::Layout& L, int sign,
+    static void run(const float* grid, float* out, gt_nufft_plan* p, int C, const gt::Layout& L, int sign,
                     const float* dcw, long long dcw_len, int accumulate, hipStream_t s) {
         const unsigned chunks = (unsigned)((C + gt::k3Coils - 1) / gt::k3Coils);
         const size_t lds = size_t(gt::c2nc3_lds_bytes(KW));
         const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
+        Binning& b = p->bc;
+        if (mode == 1) {
+            b.dcw_e.reserve(std::max<size_t>(b.n, 1) * sizeof(float));
+            hipLaunchKernelGGL(gt::entry_dcw_kernel, dim3((unsigned)((b.n + 255) / 256)), dim3(256), 0, s,
+                               b.list.as<uint32_t>(), (long long)b.n, dcw, b.dcw_e.as<float>());
+        }
+        // persistent: two workgroups per CU (their LDS), a multiple of the 8
+        // XCDs so that every item stays on the XCD its position assigns
+        const int resident = std::max(8, (2 * p->cus) & ~7);
+        const unsigned ctas = (unsigned)std::min(b.n_items, resident);
         auto k = mode == 0 ? gt::c2nc_multi3d_kernel<KW, 0>
                            : (mode == 1 ? gt::c2nc_multi3d_kernel<KW, 1> : gt::c2nc_multi3d_kernel<KW, 2>);
-        hipLaunchKernelGGL(k, dim3((unsigned)p->bc.n_items, chunks), dim3(64 * gt::k3Waves), lds, s, grid, out,
-                           p->bc.table.as<uint32_t>(), p->bc.items.as<gt::WorkItem>(), int(p->M), int(p->F), C,
-                           p->bc.geom, L, sign, dcw, dcw_len, accumulate);
+        hipLaunchKernelGGL(k, dim3(ctas, chunks), dim3(64 * gt::k3Waves), lds, s, grid, out, b.table.as<uint32_t>(),
+                           b.items.as<gt::WorkItem>(), b.n_items, int(p->M), int(p->F), C, b.geom, L, sign, dcw,
+                           dcw_len, b.dcw_e.as<float>(), accumulate);
     }
 };
 
@@ -684,11 +697,14 @@ std::vector<uint32_t> sort_pairs(DevBuf& keys, DevBuf& vals, size_t npairs, uint
     GT_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, tb, keys.as<uint32_t>(), keys_out.as<uint32_t>(),
                                               vals.as<uint32_t>(), vals_out.as<uint32_t>(), (int)npairs, 0,
                                               end_bit, s));
-    hist.reserve(size_t(nkeys) * 4);
-    GT_HIP(hipMemsetAsync(hist.ptr, 0, size_t(nkeys) * 4, s));
+    hist.reserve(size_t(nkeys) * 4 * 3);  // counts, first, last
+    uint32_t* cnt = hist.as<uint32_t>();
+    GT_HIP(hipMemsetAsync(cnt + nkeys, 0, size_t(nkeys) * 4 * 2, s));
     if (npairs)
-        hipLaunchKernelGGL(gt::histogram_keys_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s,
-                           keys_out.as<uint32_t>(), (long long)npairs, hist.as<uint32_t>());
+        hipLaunchKernelGGL(gt::run_bounds_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s,
+                           keys_out.as<uint32_t>(), (long long)npairs, cnt + nkeys, cnt + 2 * size_t(nkeys));
+    hipLaunchKernelGGL(gt::run_counts_kernel, dim3((nkeys + 255) / 256), dim3(256), 0, s, cnt + nkeys,
+                       cnt + 2 * size_t(nkeys), nkeys, cnt);
     GT_HIP(hipGetLastError());
     std::vector<uint32_t> counts(nkeys, 0);
     GT_HIP(hipMemcpyAsync(counts.data(), hist.ptr, size_t(nkeys) * 4, hipMemcpyDeviceToHost, s));
@@ -759,7 +775,9 @@ std::vector<uint32_t> bin_home(gt_nufft_plan* p, const gt::GridGeom& g, Binning&
 // one L2 -- each run is sorted longest first (the dispatcher hands out
 // workgroups in index order), and the runs are interleaved, padded with no-op
 // items so that item i always runs on XCD i mod kXcds.  GT_NUFFT_XCD=0 keeps
-// a single longest-first order instead.
+// a single longest-first order instead.  The 3D C2NC keeps key order within a
+// run (no longest-first sort): its items are short, and neighbouring tiles
+// then stage overlapping halos while those are in the XCD's L2.
 constexpr int kXcds = 8;
 
 bool xcd_ordering() {
@@ -770,8 +788,9 @@ bool xcd_ordering() {
     return on;
 }
 
-std::vector<gt::WorkItem> xcd_order(std::vector<gt::WorkItem> items) {
-    auto longest_first = [](std::vector<gt::WorkItem>& v) {
+std::vector<gt::WorkItem> xcd_order(std::vector<gt::WorkItem> items, bool sort_longest) {
+    auto longest_first = [sort_longest](std::vector<gt::WorkItem>& v) {
+        if (!sort_longest) return;
         std::stable_sort(v.begin(), v.end(), [](const gt::WorkItem& a, const gt::WorkItem& b) {
             return (a.end - a.begin) > (b.end - b.begin);
         });
@@ -832,7 +851,8 @@ void make_items(gt_nufft_plan* p, Binning& b, const std::vector<uint32_t>& count
         }
         begin += n;
     }
-    items = xcd_order(std::move(items));
+    // the 3D C2NC items (short, staging a halo each) keep key order
+    items = xcd_order(std::move(items), !(p->fast3d && !partials));
     if (items.empty()) items.push_back({0, 0, 0, gt::kNoopItem});  // keep one launchable item
     b.n_items = int(items.size());
     b.n_red = int(red.size());
@@ -1029,20 +1049,22 @@ void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layo
 void pad_deapod(gt_nufft_plan* p, const void* img, void* grid, size_t nb, bool in_os, float scale,
                 hipStream_t s) {
     gt::PadCrop pc = padcrop(p, true, in_os, scale, 1);
-    const long long total = (long long)(nb * p->G());
+    const long long rows = (long long)(nb * p->G() / p->Nos[0]);
     StageTimer timer(p, GT_STAGE_PAD, s);
-    hipLaunchKernelGGL(gt::pad_deapod_kernel, dim3(grid_blocks(total)), dim3(256), 0, s, (const float2*)img,
-                       (float2*)grid, (long long)nb, p->geom, pc);
+    hipLaunchKernelGGL(gt::pad_deapod_kernel, dim3((unsigned)((rows + gt::kRowsPerBlock - 1) / gt::kRowsPerBlock)),
+                       dim3(64, gt::kRowsPerBlock), 0, s, (const float2*)img, (float2*)grid, (long long)nb, p->geom,
+                       pc);
     GT_HIP(hipGetLastError());
 }
 
 void crop_deapod(gt_nufft_plan* p, const void* grid, void* img, size_t nb, bool out_os, float scale,
                  hipStream_t s) {
     gt::PadCrop pc = padcrop(p, false, out_os, scale, 1);
-    const long long total = (long long)(nb * (out_os ? p->G() : p->Nimg()));
+    const long long rows = (long long)(nb * (out_os ? p->G() / p->Nos[0] : p->Nimg() / p->N[0]));
     StageTimer timer(p, GT_STAGE_CROP, s);
-    hipLaunchKernelGGL(gt::crop_deapod_kernel, dim3(grid_blocks(total)), dim3(256), 0, s, (const float2*)grid,
-                       (float2*)img, (long long)nb, p->geom, pc);
+    hipLaunchKernelGGL(gt::crop_deapod_kernel, dim3((unsigned)((rows + gt::kRowsPerBlock - 1) / gt::kRowsPerBlock)),
+                       dim3(64, gt::kRowsPerBlock), 0, s, (const float2*)grid, (float2*)img, (long long)nb, p->geom,
+                       pc);
     GT_HIP(hipGetLastError());
 }
 
@@ -1311,6 +1333,7 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         if (dev < 0) GT_HIP(hipGetDevice(&dev));
         p->device = dev;
         set_device(p.get());
+        GT_HIP(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev));
         for (int i = 0; i < p->D; i++) {
             p->filt_host[i] = deapod_1d(p->Nos[i], p->radius, p->one_over_W, p->beta[i]);
             p->filt[i].reserve(p->Nos[i] * sizeof(float));
@@ -1475,9 +1498,10 @@ gt_status gt_nufft_deapodize(gt_nufft_plan* p, void* image, const size_t* dims, 
         }
         set_device(p);
         gt::PadCrop pc = padcrop(p, false, true, 1.0f, 0);
-        const long long total = (long long)(sh.nb * p->G());
-        hipLaunchKernelGGL(gt::crop_deapod_kernel, dim3(grid_blocks(total)), dim3(256), 0, S(stream),
-                           (const float2*)image, (float2*)image, (long long)sh.nb, p->geom, pc);
+        const long long rows = (long long)(sh.nb * p->G() / p->Nos[0]);  // in place: each thread rewrites what it read
+        hipLaunchKernelGGL(gt::crop_deapod_kernel, dim3((unsigned)((rows + gt::kRowsPerBlock - 1) / gt::kRowsPerBlock)),
+                           dim3(64, gt::kRowsPerBlock), 0, S(stream), (const float2*)image, (float2*)image,
+                           (long long)sh.nb, p->geom, pc);
         GT_HIP(hipGetLastError());
     });
 }
