@@ -245,6 +245,86 @@ __global__ __launch_bounds__(kRedThreads) void cg_beta_kernel(const double2* __r
 }
 
 // p = beta p + z (z = P^2 r, held in q, or r)
+// Device-side iteration it of the solve (no host reads): the state of
+// iteration it is st[it], and the step kernels below each sum the FP64 partial
+// sums of the previous kernel themselves, in the same order in every block, so
+// the alpha and beta finishers are not separate launches.  x, r update with
+// alpha from the <p, q> partials; partial sums of <r, z>.
+__device__ __forceinline__ double2 sum_partials(const double2* __restrict__ partial, int nb, double2* sh) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) {
+        acc.x += partial[i].x;
+        acc.y += partial[i].y;
+    }
+    const double2 s = block_sum(acc, sh);
+    __syncthreads();  // sh is reused by the caller
+    return s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_xr_step_kernel(float2* __restrict__ x, float2* __restrict__ r,
+                                                                 const float2* __restrict__ p, float2* __restrict__ q,
+                                                                 CgParams P, const CgState* __restrict__ st,
+                                                                 const double2* __restrict__ pq, int nb,
+                                                                 double2* __restrict__ partial) {
+    __shared__ double2 sh[kRedThreads];
+    const double2 s = sum_partials(pq, nb, sh);
+    const double m = s.x * s.x + s.y * s.y;
+    float ar = 0.0f, ai = 0.0f;
+    if (!st->done && m > 0.0) {  // cg_alpha_kernel
+        ar = (float)(st->rq * s.x / m);
+        ai = (float)(-st->rq * s.y / m);
+    }
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < P.n; i += (long long)gridDim.x * kRedThreads) {
+        const float2 pv = p[i], qv = q[i];
+        float2 xv = x[i], rv = r[i];
+        xv.x += ar * pv.x - ai * pv.y;
+        xv.y += ar * pv.y + ai * pv.x;
+        rv.x -= ar * qv.x - ai * qv.y;
+        rv.y -= ar * qv.y + ai * qv.x;
+        x[i] = xv;
+        r[i] = rv;
+        float2 z = rv;
+        if (P.w) {
+            z = precond2(P, i, rv);
+            q[i] = z;
+        }
+        acc.x += (double)rv.x * z.x + (double)rv.y * z.y;
+    }
+    const double2 t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// p = z + beta p with beta from the <r, z> partials; block 0 writes the state
+// of the next iteration (cg_beta_kernel).
+__global__ __launch_bounds__(kRedThreads) void cg_p_step_kernel(float2* __restrict__ p, const float2* __restrict__ z,
+                                                                long long n, const CgState* __restrict__ st,
+                                                                CgState* __restrict__ next,
+                                                                const double2* __restrict__ rz, int nb,
+                                                                double* __restrict__ hist, int it, double tol) {
+    __shared__ double2 sh[kRedThreads];
+    const double rr = sum_partials(rz, nb, sh).x;
+    const CgState c = *st;
+    const double beta = c.done ? 0.0 : (c.rq > 0.0 ? rr / c.rq : 0.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CgState nx = c;
+        nx.alpha_re = nx.alpha_im = 0.0;
+        nx.beta = beta;
+        nx.it = it + 1;
+        if (!c.done) {
+            nx.rq = rr;
+            if (rr / c.rq0 < tol || !(rr > 0.0)) nx.done = 1;
+        }
+        *next = nx;
+        if (hist) hist[it + 1] = c.done ? 0.0 : rr;
+    }
+    const float b = (float)beta;
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kRedThreads) {
+        const float2 pv = p[i], zv = z[i];
+        p[i] = make_float2(b * pv.x + zv.x, b * pv.y + zv.y);
+    }
+}
+
 __global__ void cg_p_kernel(float2* __restrict__ p, const float2* __restrict__ z, long long n,
                             const CgState* __restrict__ st) {
     const float b = (float)st->beta;
@@ -341,8 +421,8 @@ struct gt_sense_op {
         p.reserve(bytes);
         q.reserve(bytes);
         mhd.reserve(bytes);
-        partial.reserve(kRedBlocks * sizeof(double2));
-        state.reserve(sizeof(CgState));
+        partial.reserve(2 * kRedBlocks * sizeof(double2));
+        state.reserve(size_t(iterations + 1) * sizeof(CgState));  // one per iteration (device-side solve)
         hist.reserve(size_t(iterations + 1) * sizeof(double));
     }
 
@@ -401,10 +481,23 @@ void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, b
                        op->r.as<float2>(), op->p.as<float2>(), P, part);
     hipLaunchKernelGGL(cg_init_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st, hist);
     GT_HIP(hipGetLastError());
-    if (host_sync) {
-        host_read(op, &h, s);
-        if (h.done) return;
+    if (!host_sync) {
+        // device-side: iteration it reads state it and writes state it + 1
+        for (int it = 0; it < d->iterations; it++) {
+            op->mult_MH_M(op->p.ptr, op->q.ptr, false, s);
+            hipLaunchKernelGGL(cg_q_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->p.as<float2>(),
+                               op->q.as<float2>(), P, part);
+            hipLaunchKernelGGL(cg_xr_step_kernel, dim3(nb), dim3(kRedThreads), 0, s, xf, op->r.as<float2>(),
+                               op->p.as<float2>(), op->q.as<float2>(), P, st + it, part, int(nb), part + nb);
+            const float2* z = P.w ? op->q.as<float2>() : op->r.as<float2>();
+            hipLaunchKernelGGL(cg_p_step_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->p.as<float2>(), z, P.n,
+                               st + it, st + it + 1, part + nb, int(nb), hist, it, P.tol);
+            GT_HIP(hipGetLastError());
+        }
+        return;
     }
+    host_read(op, &h, s);
+    if (h.done) return;
     for (int it = 0; it < d->iterations; it++) {
         // q = (w_E E^H E + kappa I) p
         op->mult_MH_M(op->p.ptr, op->q.ptr, false, s);
@@ -508,8 +601,9 @@ void build_graph(gt_cg_batch* b, int nbr, hipGraph_t* graph, hipGraphExec_t* exe
 void tune_graph(gt_cg_batch* b, hipStream_t s) {
     const int n = int(b->ops.size());
     std::vector<int> cand;
-    for (int c : {2, 4, 8, 16, 32})
-        if (c <= n) cand.push_back(c);
+    for (int c : {2, 4, 8, 16, 32, 64})
+        if (c < n) cand.push_back(c);
+    cand.push_back(n);  // one branch per problem
     if (cand.empty()) cand.push_back(1);
     double best = 1e300;
     for (int nbr : cand) {
@@ -519,6 +613,7 @@ void tune_graph(gt_cg_batch* b, hipStream_t s) {
         GT_HIP(hipGraphLaunch(e, s));
         GT_HIP(hipStreamSynchronize(s));
         const auto t0 = std::chrono::steady_clock::now();
+        GT_HIP(hipGraphLaunch(e, s));
         GT_HIP(hipGraphLaunch(e, s));
         GT_HIP(hipStreamSynchronize(s));
         const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -14,9 +14,10 @@
 //   A persistent workgroup stages one 4 x 4 x 4 tile plus the kernel halo
 //   for 8 coils in LDS, one plane per coil in the grid's own order, so the
 //   staging loads read whole grid rows; the next item's staging is in flight
-//   while the current one is gathered.  Each wavefront gathers two samples at
-//   a time: lane (sample slot, x half, coil, re/im) sums its half of every x
-//   row of the window, and the two halves are added across lanes.
+//   while the current one is gathered.  Each wavefront gathers four samples at
+//   a time: lane (sample slot, x half, coil) reads whole complex cells and
+//   sums its half of every x row of the window; the halves are added across
+//   lanes.
 #pragma once
 
 #include "gt_kernels_reg.h"
@@ -292,7 +293,8 @@ __global__ __launch_bounds__(256) void nc2c_reduce3d_kernel(const float* __restr
 constexpr int k3S0 = 4, k3S1 = 4, k3S2 = 4;  // staged tile (x, y, z), plus the kernel halo
 constexpr int k3CDw = 24;                    // dwords per C2NC table entry
 constexpr int k3XSlots = 8;                  // x weight slots (two halves of up to 4)
-constexpr int k3Seg = 64;                    // entries per C2NC work item (its table is staged)
+constexpr int k3Chunk = 64;                  // C2NC table entries staged in LDS at a time
+constexpr int k3Seg = 512;                   // entries per C2NC work item
 constexpr int k3Waves = 8;                   // wavefronts per C2NC workgroup
 constexpr int k3Cell = 2 * k3Coils;          // LDS floats per staged cell
 
@@ -333,9 +335,10 @@ __global__ void c2nc_table3d_kernel(const float* __restrict__ p, const uint32_t*
 // E0 x E1 x E2 (tile + KW - 1 per dimension) as (re, im) pairs in the grid's
 // own order (x fastest), then zeroed pad cells -- a window reads up to one cell,
 // row and plane past the staged tile (and, with the x halves, 2 XH - KW more
-// cells), always with weight 0.  The plane stride is 4 (mod 32) floats: the 16
-// (coil, re/im) lanes of one x half then fall on banks {4c + comp}, and the
-// other half, 2 XH floats on (XH = 3 for W 5.5), on the remaining 16 banks.
+// cells), always with weight 0.  The plane stride is 4 (mod 32) floats: the 8
+// coil lanes of one x half, reading (re, im), then fall on banks {4c, 4c + 1},
+// and those of the other half, 2 XH floats on (XH = 3 for W 5.5), on the
+// remaining 16 banks.
 __host__ __device__ constexpr int c2nc3_cells(int KW) { return (k3S0 + KW - 1) * (k3S1 + KW - 1) * (k3S2 + KW - 1); }
 __host__ __device__ constexpr int c2nc3_pad_cells(int KW) {
     return (k3S0 + KW - 1) * (k3S1 + KW - 1) + (k3S0 + KW - 1) + 8;
@@ -344,13 +347,14 @@ __host__ __device__ constexpr int c2nc3_plane_floats(int KW) {
     return ((2 * (c2nc3_cells(KW) + c2nc3_pad_cells(KW)) + 31) / 32) * 32 + 4;
 }
 __host__ __device__ constexpr int c2nc3_lds_floats(int KW) { return k3Coils * c2nc3_plane_floats(KW); }
-__host__ __device__ constexpr int c2nc3_lds_bytes(int KW) { return (c2nc3_lds_floats(KW) + k3Seg * k3CDw + k3Seg) * 4; }
+__host__ __device__ constexpr int c2nc3_lds_bytes(int KW) { return (c2nc3_lds_floats(KW) + k3Chunk * k3CDw + k3Chunk) * 4; }
 
-// Staging of one work item (tile + halo for 8 coils, and its table) into
-// registers.  Wave w stages coil c0 + w; its lanes (row r, float f) cover R
-// grid rows of 2 E0 floats at a time, so a load instruction reads R runs of
-// 2 E0 consecutive floats and the LDS stores are contiguous.  Row and plane
-// wraps and centring signs are worked out once per row and plane.
+// Staging of one work item into registers: its tile + halo for 8 coils, and
+// one chunk (k3Chunk entries) of its table.  Wave w stages coil c0 + w; its
+// lanes (row r, float f) cover R grid rows of 2 E0 floats at a time, so a load
+// instruction reads R runs of 2 E0 consecutive floats and the LDS stores are
+// contiguous.  Row and plane wraps and centring signs are worked out once per
+// row and plane.
 template <int KW>
 struct C2NC3Stage {
     static constexpr int E0 = k3S0 + KW - 1, E1 = k3S1 + KW - 1, E2 = k3S2 + KW - 1;
@@ -359,26 +363,15 @@ struct C2NC3Stage {
     static constexpr int R = kLanes / RF;           // rows per load instruction
     static constexpr int kYI = (E1 + R - 1) / R;    // load instructions per plane
     static constexpr int kU4 = k3CDw / 4;           // table uint4 per entry
-    static_assert(k3Seg * kU4 <= NT, "one table uint4 per thread");
+    static_assert(k3Chunk * kU4 <= NT, "one table uint4 per thread");
     static_assert(k3Waves == k3Coils, "one wave per coil");
     float v[E2][kYI];
     int par;  // bit m * E2 + ez: centring sign of value (ez, m)
     uint4 ta;
-    float dj;  // dcw of entry tid (per-entry dcw array, or 1)
-    uint32_t key, ne;
-    bool noop;
+    float dj;  // dcw of chunk entry tid (per-entry dcw array, or 1)
 
-    __device__ __forceinline__ void load(const float* __restrict__ grid, const uint32_t* __restrict__ tab,
-                                         const float* __restrict__ dcw_e, const WorkItem& it, int tid, int c0,
-                                         int C, int F, const GridGeom& g, const Layout& L, int sign) {
-        noop = it.partial == kNoopItem;
-        key = it.key;
-        ne = noop ? 0u : it.end - it.begin;
-        const int nt4 = (int)ne * kU4;
-        ta = make_uint4(0u, 0u, 0u, 0u);
-        if (tid < nt4) ta = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * kU4 + tid];
-        dj = 1.0f;
-        if (dcw_e && tid < (int)ne) dj = dcw_e[it.begin + tid];
+    __device__ __forceinline__ void load_grid(const float* __restrict__ grid, uint32_t key, bool live, int tid,
+                                              int c0, int C, int F, const GridGeom& g, const Layout& L, int sign) {
         const int fr = (int)floorf(g.radius);
         const int f = (int)(key / (uint32_t)g.ntiles_total);
         int org[3];
@@ -387,7 +380,7 @@ struct C2NC3Stage {
         const int lane = tid & 63, coil = c0 + (tid >> 6);
         const int r = lane / RF, fl = lane % RF;  // row in the instruction, float in the row
         const int ex = fl >> 1, comp = fl & 1;
-        const bool cok = !noop && r < R && coil < C;
+        const bool cok = live && r < R && coil < C;
         int gx = sx + ex;
         const int px = gx & 1;
         gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
@@ -418,7 +411,16 @@ struct C2NC3Stage {
         }
     }
 
-    __device__ __forceinline__ void store(float* sg, uint32_t* st, float* sd, int tid) const {
+    // entries [e0, e0 + n) of the table, n <= k3Chunk
+    __device__ __forceinline__ void load_table(const uint32_t* __restrict__ tab, const float* __restrict__ dcw_e,
+                                               uint32_t e0, int n, int tid) {
+        ta = make_uint4(0u, 0u, 0u, 0u);
+        if (tid < n * kU4) ta = reinterpret_cast<const uint4*>(tab)[(size_t)e0 * kU4 + tid];
+        dj = 1.0f;
+        if (dcw_e && tid < n) dj = dcw_e[e0 + tid];
+    }
+
+    __device__ __forceinline__ void store_grid(float* sg, int tid) const {
         const int lane = tid & 63;
         const int r = lane / RF, fl = lane % RF;
         if (r < R) {
@@ -432,8 +434,11 @@ struct C2NC3Stage {
                     d[(ez * E1 + m * R) * RF] = ((par >> (m * E2 + ez)) & 1) ? -x : x;
                 }
         }
-        if (tid < (int)ne * kU4) reinterpret_cast<uint4*>(st)[tid] = ta;
-        if (tid < (int)ne) sd[tid] = dj;
+    }
+
+    __device__ __forceinline__ void store_table(uint32_t* st, float* sd, int n, int tid) const {
+        if (tid < n * kU4) reinterpret_cast<uint4*>(st)[tid] = ta;
+        if (tid < n) sd[tid] = dj;
     }
 };
 
@@ -448,14 +453,15 @@ __global__ void entry_dcw_kernel(const uint32_t* __restrict__ list, long long n,
 // MODE 0: no dcw, no accumulation; 1: one dcw weight per sample (dcw_e: per
 // table entry), no accumulation; 2: anything else.
 // Persistent: workgroup b takes items b, b + gridDim.x, ... (gridDim.x a
-// multiple of 8 keeps each item on its XCD).  While it gathers one item, the
-// next item's staging loads are in flight in registers.
+// multiple of 8 keeps each item on its XCD).  An item stages its tile once and
+// walks its table in chunks of k3Chunk entries; the next chunk, or after the
+// last one the next item's staging, is in flight while a chunk is gathered.
 template <int KW, int MODE>
 __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
     const float* __restrict__ grid, float* __restrict__ out, const uint32_t* __restrict__ tab,
     const WorkItem* __restrict__ items, int n_items, int M, int F, int C, GridGeom g, Layout L, int sign,
     const float* __restrict__ dcw, long long dcw_len, const float* __restrict__ dcw_e, int accumulate) {
-    extern __shared__ float sg[];  // coil planes, then the item's table, then its dcw
+    extern __shared__ float sg[];  // coil planes, then a table chunk, then its dcw
     using S = C2NC3Stage<KW>;
     constexpr int E0 = S::E0, E1 = S::E1;
     constexpr int NCELL = c2nc3_cells(KW);
@@ -465,7 +471,7 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
     const int lane = tid & 63, wave = wave_uniform(tid >> 6);
     const int c0 = blockIdx.y * k3Coils;
     uint32_t* st = reinterpret_cast<uint32_t*>(sg + c2nc3_lds_floats(KW));
-    float* sd = reinterpret_cast<float*>(st + k3Seg * k3CDw);
+    float* sd = reinterpret_cast<float*>(st + k3Chunk * k3CDw);
     const long long FM = (long long)F * M;
 
     for (int i = tid; i < k3Coils * (PS - 2 * NCELL); i += S::NT) {
@@ -473,77 +479,117 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
         sg[c * PS + 2 * NCELL + o] = 0.0f;
     }
 
-    // lane = (slot, x half, coil, re/im); lanes past the last coil mirror coil
-    // c0 (same values, same output address)
-    const int slot = lane >> 5, xh = (lane >> 4) & 1, cc = lane & 15;
-    const int coil = c0 + (cc >> 1);
+    // lane = (slot, x half, coil): a lane reads whole cells (re, im); lanes past
+    // the last coil mirror coil c0 (same values, same output address)
+    const int slot = lane >> 4, xh = (lane >> 3) & 1, cl = lane & 7;
+    const int coil = c0 + cl;
     const bool on = coil < C;
     const int lcoil = on ? coil : c0;
-    const int lcc = on ? cc : (cc & 1);
-    float* ol = out + 2 * (long long)lcoil * FM + (cc & 1);
-    const float* sl = sg + (lcc >> 1) * PS + (lcc & 1) + 2 * xh * XH;
+    float2* ol = reinterpret_cast<float2*>(out) + (long long)lcoil * FM;
+    const float2* sl = reinterpret_cast<const float2*>(sg + (on ? cl : 0) * PS) + xh * XH;
 
     const float* de = MODE == 1 ? dcw_e : nullptr;
+    auto entries = [](const WorkItem& w) -> uint32_t { return w.partial == kNoopItem ? 0u : w.end - w.begin; };
+    auto chunk_n = [](uint32_t ne, uint32_t ch) -> int {
+        return (int)min((uint32_t)k3Chunk, ne - min(ne, ch * (uint32_t)k3Chunk));
+    };
     S cur;
     int item = blockIdx.x;
-    if (item < n_items) cur.load(grid, tab, de, items[item], tid, c0, C, F, g, L, sign);
+    if (item < n_items) {
+        const WorkItem it = items[item];
+        const uint32_t ne = entries(it);
+        cur.load_grid(grid, it.key, ne > 0, tid, c0, C, F, g, L, sign);
+        cur.load_table(tab, de, it.begin, chunk_n(ne, 0), tid);
+    }
     for (; item < n_items; item += gridDim.x) {
-        __syncthreads();  // the previous item's gather is done with LDS
-        cur.store(sg, st, sd, tid);
-        __syncthreads();
-        const uint32_t ne = cur.ne;
-        const bool noop = cur.noop;
-        // the next item's staging is in flight during this item's gather
+        const WorkItem it = items[item];
+        const uint32_t ne = entries(it);
+        const uint32_t nch = (ne + k3Chunk - 1) / k3Chunk;
         const int nxt = item + (int)gridDim.x;
-        if (nxt < n_items) cur.load(grid, tab, de, items[nxt], tid, c0, C, F, g, L, sign);
-        if (noop) continue;
-        // wave w takes a contiguous share of the entries, two at a time
-        const uint32_t q0 = (ne * (uint32_t)wave) / k3Waves, q1 = (ne * (uint32_t)(wave + 1)) / k3Waves;
-        for (uint32_t k = q0; k < q1; k += 2) {
-            const uint32_t e = min(k + (uint32_t)slot, q1 - 1);  // an odd tail repeats the last entry
-            const uint32_t* te = st + e * k3CDw;
-            const uint32_t j = te[0];
-            const uint32_t pk = te[1];
-            const int bx = (int)(pk & 0xff), by = (int)((pk >> 8) & 0xff), bz = (int)((pk >> 16) & 0xff);
-            float wx[XH], wy[KW], wz[KW];
-#pragma unroll
-            for (int i = 0; i < XH; i++) wx[i] = __uint_as_float(te[2 + xh * XH + i]);
-#pragma unroll
-            for (int i = 0; i < KW; i++) {
-                wy[i] = __uint_as_float(te[10 + i]);
-                wz[i] = __uint_as_float(te[17 + i]);
+        // issues the loads of chunk ch of this item, or (past its last chunk)
+        // the next item's staging and first chunk
+        auto issue = [&](uint32_t ch) {
+            if (ch < nch) {
+                cur.load_table(tab, de, it.begin + ch * k3Chunk, chunk_n(ne, ch), tid);
+            } else if (nxt < n_items) {
+                const WorkItem in = items[nxt];
+                const uint32_t nn = entries(in);
+                cur.load_grid(grid, in.key, nn > 0, tid, c0, C, F, g, L, sign);
+                cur.load_table(tab, de, in.begin, chunk_n(nn, 0), tid);
             }
-            const float* base = sl + 2 * ((bz * E1 + by) * E0 + bx);
-            float a = 0.0f;
+        };
+        __syncthreads();  // the previous item's gather is done with LDS
+        cur.store_grid(sg, tid);
+        cur.store_table(st, sd, chunk_n(ne, 0), tid);
+        __syncthreads();
+        issue(1);
+        for (uint32_t ch = 0; ch < nch; ch++) {
+            const uint32_t n = (uint32_t)chunk_n(ne, ch);
+            // wave w takes a contiguous share of the chunk, four entries at a time
+            const uint32_t q0 = (n * (uint32_t)wave) / k3Waves, q1 = (n * (uint32_t)(wave + 1)) / k3Waves;
+            for (uint32_t k = q0; k < q1; k += 4) {
+                const uint32_t e = min(k + (uint32_t)slot, q1 - 1);  // a short tail repeats the last entry
+                const uint32_t* te = st + e * k3CDw;
+                const uint32_t j = te[0];
+                const uint32_t pk = te[1];
+                const int bx = (int)(pk & 0xff), by = (int)((pk >> 8) & 0xff), bz = (int)((pk >> 16) & 0xff);
+                float wx[XH], wy[KW], wz[KW];
 #pragma unroll
-            for (int iz = 0; iz < KW; iz++) {
-                float v[KW][XH];
+                for (int i = 0; i < XH; i++) wx[i] = __uint_as_float(te[2 + xh * XH + i]);
 #pragma unroll
-                for (int iy = 0; iy < KW; iy++)
-#pragma unroll
-                    for (int ix = 0; ix < XH; ix++) v[iy][ix] = base[2 * ((iz * E1 + iy) * E0 + ix)];
-                float pz = 0.0f;
-#pragma unroll
-                for (int iy = 0; iy < KW; iy++) {
-                    float r = 0.0f;
-#pragma unroll
-                    for (int ix = 0; ix < XH; ix++) r += wx[ix] * v[iy][ix];
-                    pz += wy[iy] * r;
+                for (int i = 0; i < KW; i++) {
+                    wy[i] = __uint_as_float(te[10 + i]);
+                    wz[i] = __uint_as_float(te[17 + i]);
                 }
-                a += wz[iz] * pz;
+                const float2* base = sl + ((bz * E1 + by) * E0 + bx);
+                float ar = 0.0f, ai = 0.0f;
+#pragma unroll
+                for (int iz = 0; iz < KW; iz++) {
+                    float2 v[KW][XH];
+#pragma unroll
+                    for (int iy = 0; iy < KW; iy++)
+#pragma unroll
+                        for (int ix = 0; ix < XH; ix++) v[iy][ix] = base[(iz * E1 + iy) * E0 + ix];
+                    float pr = 0.0f, pi = 0.0f;
+#pragma unroll
+                    for (int iy = 0; iy < KW; iy++) {
+                        float rr = 0.0f, ri = 0.0f;
+#pragma unroll
+                        for (int ix = 0; ix < XH; ix++) {
+                            rr += wx[ix] * v[iy][ix].x;
+                            ri += wx[ix] * v[iy][ix].y;
+                        }
+                        pr += wy[iy] * rr;
+                        pi += wy[iy] * ri;
+                    }
+                    ar += wz[iz] * pr;
+                    ai += wz[iz] * pi;
+                }
+                // add the other x half (lane ^ 8)
+                ar += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(ar), 0x201F));
+                ai += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(ai), 0x201F));
+                if (xh == 0) {
+                    float d = 1.0f;
+                    if constexpr (MODE == 1) d = sd[e];
+                    if constexpr (MODE == 2) {
+                        const int dm = dcw_mode(dcw, dcw_len, FM);
+                        if (dm == 1) d = dcw[j];
+                        else if (dm == 2) d = dcw[dcw_index(lcoil, j, FM, dcw_len)];
+                    }
+                    float2 a = make_float2(ar * d, ai * d);
+                    if (MODE == 2 && accumulate) {
+                        const float2 old = ol[j];
+                        a.x += old.x;
+                        a.y += old.y;
+                    }
+                    ol[j] = a;
+                }
             }
-            // add the other x half (lane ^ 16)
-            a += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(a), 0x401F));
-            if (xh == 0) {
-                if constexpr (MODE == 1) a *= sd[e];
-                if constexpr (MODE == 2) {
-                    const int dm = dcw_mode(dcw, dcw_len, FM);
-                    if (dm == 1) a *= dcw[j];
-                    else if (dm == 2) a *= dcw[dcw_index(lcoil, j, FM, dcw_len)];
-                }
-                const long long o = 2 * (long long)j;
-                if (MODE == 2 && accumulate) a += ol[o];
-                ol[o] = a;
+            if (ch + 1 < nch) {
+                __syncthreads();  // the chunk's gather is done with the table
+                cur.store_table(st, sd, chunk_n(ne, ch + 1), tid);
+                __syncthreads();
+                issue(ch + 2);
             }
         }
     }

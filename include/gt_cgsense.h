@@ -82,7 +82,7 @@ gt_status gt_cgsense_batch_create(gt_cg_batch** batch, gt_sense_op* const* ops, 
 gt_status gt_cgsense_batch_run(gt_cg_batch* batch, gt_stream_t stream);
 gt_status gt_cgsense_batch_destroy(gt_cg_batch* batch);
 /* Parallel branches of a batched graph, picked at creation by timing one
- * replay per candidate (2, 4, 8, 16, 32); 0 for GT_CG_PLAIN. */
+ * replay per candidate (2, 4, ..., 64, and one per problem); 0 for GT_CG_PLAIN. */
 gt_status gt_cgsense_batch_branches(const gt_cg_batch* batch, int* branches);
 
 #ifdef __cplusplus
