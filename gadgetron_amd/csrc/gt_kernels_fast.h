@@ -207,15 +207,19 @@ __device__ __forceinline__ int dcw_mode(const float* dcw, long long dcw_len, lon
 // 2 floor(r)), so reading all KW x KW slots reaches at most one row and one
 // cell past the staged tile, always in slots of weight 0.
 // Rounded up to whole float4s: the work item's table entries follow it.
-__host__ __device__ inline int c2nc_lds_floats(int T0, int T1, int KW) {
-    return (((T1 + KW) * (T0 + KW - 1) + 1) * kSubP + 3) & ~3;
+// A staged cell holds the 2 x 32 / G floats of a wave's coils plus one (an
+// odd stride keeps rows conflict-free).
+__host__ __device__ constexpr int c2nc_cell_stride(int G) { return 2 * (kCoilsPerWave / G) + 1; }
+
+__host__ __device__ inline int c2nc_lds_floats(int T0, int T1, int KW, int G = 1) {
+    return (((T1 + KW) * (T0 + KW - 1) + 1) * c2nc_cell_stride(G) + 3) & ~3;
 }
 
 // C2NC work items hold at most this many entries (their table is staged in LDS).
 constexpr int kC2NCSeg = 64;
 
-__host__ __device__ inline int c2nc_lds_bytes(int KW) {
-    return (c2nc_lds_floats(8, 8, KW) + kC2NCSeg * kTabDw) * 4;
+__host__ __device__ inline int c2nc_lds_bytes(int KW, int G = 1) {
+    return (c2nc_lds_floats(8, 8, KW, G) + kC2NCSeg * kTabDw + kC2NCSeg) * 4;
 }
 
 // ---------------------------------------------------------------------------

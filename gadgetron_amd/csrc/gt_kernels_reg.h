@@ -56,12 +56,22 @@ __device__ __forceinline__ void nc2c_chunk_load(const float* __restrict__ wt, co
 
 // DM: how dcw is applied (dcw_mode): 0 none, 1 per sample (scalar loads),
 // 2 per coil element (loaded with the samples).
-template <int DM>
+// G: coil groups per wave (1, 2 or 4) for 32 / G coils.  Group g keeps its own
+// copy of the tile and takes table entries g, g + G, ... of every chunk (rows
+// follow the union of the groups' row masks; a row outside an entry's window
+// has weight 0); the groups' tiles are added in a fixed order at the end, so
+// the result stays bitwise repeatable.
+template <int DM, int G>
 __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
     const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
     const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
     int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
     int accumulate) {
+    static_assert(G == 1 || G == 2 || G == 4, "coil groups");
+    constexpr int CW = kCoilsPerWave / G;  // coils per wave
+    constexpr int LC = CW / 4;             // coils per lane in a chunk's sample loads
+    constexpr int XR = 2 * CW + 2;         // floats per entry of the sample transpose (+2: banks)
+    constexpr int EK = kNK / G;            // entries per group and chunk
     // LDS: the gridding loop's buffers and the write-out's transpose buffer are
     // never live together (the write-out starts with a barrier)
     __shared__ float4 smem[kNC2CLdsFloats / 4];
@@ -72,13 +82,15 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
     const int lane = threadIdx.x;
     const WorkItem it = items[blockIdx.x];
     if (it.partial == kNoopItem) return;
-    const int c0 = blockIdx.y * kCoilsPerWave;
+    const int c0 = blockIdx.y * CW;
     const long long FM = (long long)F * M;
     const uint32_t nch = (it.end - it.begin + kNK - 1) / kNK;
-    // sample loads: lane = (entry kk, coil quarter cq), 8 coils each, so one
+    // sample loads: lane = (entry kk, coil quarter cq), LC coils each, so one
     // load instruction reads 4 coils x 16 entries -- runs of consecutive
     // samples of a spoke are contiguous per coil
     const int kk = lane & (kNK - 1), cq = lane >> 4;
+    // gridding: lane = (group, coil, re/im)
+    const int grp = lane / (2 * CW), q = lane % (2 * CW);
     const float2* s2 = reinterpret_cast<const float2*>(samples);
 
     float acc[kRT * kRT];
@@ -89,13 +101,13 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
     // LDS transpose, the table of chunk c + 1 moves to LDS buffer (c + 1) & 1,
     // the table of chunk c + 2 and the samples of chunk c + 1 are issued, and
     // chunk c is gridded from LDS while those loads are in flight.
-    constexpr int kDV = DM == 2 ? kLoadCoils : 1;  // dcw per loaded coil (DM 2) or per entry (DM 1)
+    constexpr int kDV = DM == 2 ? LC : 1;  // dcw per loaded coil (DM 2) or per entry (DM 1)
     float4 wreg;
     uint32_t hreg;
-    float2 sv[kLoadCoils];
+    float2 sv[LC];
     float dv[kDV];
 #pragma unroll
-    for (int u = 0; u < kLoadCoils; u++) sv[u] = make_float2(0.0f, 0.0f);
+    for (int u = 0; u < LC; u++) sv[u] = make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int u = 0; u < kDV; u++) dv[u] = 1.0f;
 
@@ -106,8 +118,8 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
     auto issue = [&](int b) {
         const uint32_t j = sh[b][2 * kk];
 #pragma unroll
-        for (int u = 0; u < kLoadCoils; u++) {
-            int c = c0 + cq * kLoadCoils + u;
+        for (int u = 0; u < LC; u++) {
+            int c = c0 + cq * LC + u;
             c = c < C ? c : c0;  // coils past the last one repeat a valid coil, never used
             sv[u] = s2[(long long)c * FM + j];
             if constexpr (DM == 2) dv[u] = dcw[dcw_index(c, j, FM, dcw_len)];
@@ -116,35 +128,39 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
     };
     auto transpose = [&]() {
         // lane = (entry, coil quarter) -> [entry][coil][re, im]; the gridding
-        // reads lane = (coil, re/im) of entry k
+        // reads lane = (group, coil, re/im) of its entry
 #pragma unroll
-        for (int u = 0; u < kLoadCoils; u++) {
+        for (int u = 0; u < LC; u++) {
             float2 v = sv[u];
             if constexpr (DM != 0) {
                 const float d = dv[DM == 2 ? u : 0];
                 v.x *= d;
                 v.y *= d;
             }
-            *reinterpret_cast<float2*>(&xs[kk * kXsRow + 2 * (cq * kLoadCoils + u)]) = v;
+            *reinterpret_cast<float2*>(&xs[kk * XR + 2 * (cq * LC + u)]) = v;
         }
     };
     auto grid_chunk = [&](int b) {
         const int hv = (int)sh[b][lane & 31];
         // entry k + 1's weights and sample are read while entry k is gridded
-        float4 n0 = sw[b][0], n1 = sw[b][1], n2 = sw[b][2], n3 = sw[b][3];
-        float nsk = xs[lane];
+        int e = grp;  // this lane's entry of step k: k * G + grp
+        float4 n0 = sw[b][4 * e], n1 = sw[b][4 * e + 1], n2 = sw[b][4 * e + 2], n3 = sw[b][4 * e + 3];
+        float nsk = xs[e * XR + q];
 #pragma unroll
-        for (int k = 0; k < kNK; k++) {
+        for (int k = 0; k < EK; k++) {
             const float4 x0 = n0, x1 = n1, y0 = n2, y1 = n3;
             const float sk = nsk;
-            if (k + 1 < kNK) {
-                n0 = sw[b][4 * (k + 1)];
-                n1 = sw[b][4 * (k + 1) + 1];
-                n2 = sw[b][4 * (k + 1) + 2];
-                n3 = sw[b][4 * (k + 1) + 3];
-                nsk = xs[(k + 1) * kXsRow + lane];
+            if (k + 1 < EK) {
+                e = (k + 1) * G + grp;
+                n0 = sw[b][4 * e];
+                n1 = sw[b][4 * e + 1];
+                n2 = sw[b][4 * e + 2];
+                n3 = sw[b][4 * e + 3];
+                nsk = xs[e * XR + q];
             }
-            const uint32_t rows = (uint32_t)__builtin_amdgcn_readlane(hv, 2 * k + 1);
+            uint32_t rows = 0;  // rows any group's entry covers
+#pragma unroll
+            for (int g = 0; g < G; g++) rows |= (uint32_t)__builtin_amdgcn_readlane(hv, 2 * (k * G + g) + 1);
             const float wc[kRT] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
             const float wr[kRT] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
             const float s = sk;  // dcw already applied in the transpose
@@ -178,8 +194,18 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
         }
     }
 
+    // the groups' tiles, added in a fixed order: (g0 + g1) + (g2 + g3)
+    if constexpr (G == 4) {
+#pragma unroll
+        for (int i = 0; i < kRT * kRT; i++) acc[i] += __shfl_xor(acc[i], 16);
+    }
+    if constexpr (G >= 2) {
+#pragma unroll
+        for (int i = 0; i < kRT * kRT; i++) acc[i] += __shfl_xor(acc[i], 32);
+    }
+
     // write-out through LDS, half a tile (4 rows) at a time: lanes run along x
-    // within (coil, y) rows for coalesced stores
+    // within (coil, y) rows for coalesced stores (group 0 lanes hold the sums)
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     const int t = (int)(it.key % (uint32_t)g.ntiles_total);
     const int ox = (t % g.ntiles[0]) * kRT, oy = (t / g.ntiles[0]) * kRT;
@@ -192,7 +218,7 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
 #pragma unroll
         for (int c = 0; c < kRT * kRT / 2; c++) xp[c * kSubP + lane] = acc[half * (kRT * kRT / 2) + c];
         __syncthreads();
-        for (int r = lr; r < kCoilsPerWave * (kRT / 2); r += 4) {
+        for (int r = lr; r < CW * (kRT / 2); r += 4) {
             const int cl = r >> 2, ly = half * (kRT / 2) + (r & 3);
             const int cc = c0 + cl;
             const int gy = oy + ly;
@@ -247,27 +273,34 @@ __global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restr
 
 // MODE 0: no dcw, no accumulation; 1: one dcw weight per sample, no
 // accumulation; 2: anything else (per-element dcw, accumulation).
+// G: samples gathered at a time per wave (1, 2 or 4) for 32 / G coils; lane =
+// (sample slot, coil, re/im).  With G > 1 a per-sample dcw comes from dcw_e
+// (per table entry), staged with the table.
 // LDS holds at most 3 workgroups per CU (3 waves per SIMD): the waves-per-EU
 // range lets the scheduler keep a whole window's reads, and the staging's
 // loads, in flight.
-template <int KW, int MODE>
+template <int KW, int MODE, int G>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 3))) void c2nc_multi2d_kernel(
     const float* __restrict__ grid, float* __restrict__ out, const uint32_t* __restrict__ tab,
     const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g, Layout L, int sign,
-    const float* __restrict__ dcw, long long dcw_len, int accumulate) {
-    extern __shared__ float sg[];  // staged grid (c2nc_lds_floats), then the item's table
+    const float* __restrict__ dcw, long long dcw_len, const float* __restrict__ dcw_e, int accumulate) {
+    extern __shared__ float sg[];  // staged grid (c2nc_lds_floats), then the item's table (and dcw)
+    static_assert(G == 1 || G == 2 || G == 4, "sample slots");
+    constexpr int CW = kCoilsPerWave / G;  // coils per wave
+    constexpr int SP = c2nc_cell_stride(G);  // LDS floats per staged cell
     constexpr int E0 = kRT + KW - 1, E1 = kRT + KW - 1;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = wave_uniform(tid >> 6);
     const WorkItem it = items[blockIdx.x];
     if (it.partial == kNoopItem) return;
-    const int c0 = blockIdx.y * kCoilsPerWave;
+    const int c0 = blockIdx.y * CW;
     const int fr = (int)floorf(g.radius);
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     const int t = (int)(it.key % (uint32_t)g.ntiles_total);
     const int sx = (t % g.ntiles[0]) * kRT - fr, sy0 = (t / g.ntiles[0]) * kRT - fr;
     const uint32_t ne = it.end - it.begin;  // at most kC2NCSeg
-    uint4* st = reinterpret_cast<uint4*>(sg + c2nc_lds_floats(kRT, kRT, KW));
+    uint4* st = reinterpret_cast<uint4*>(sg + c2nc_lds_floats(kRT, kRT, KW, G));
+    float* sde = reinterpret_cast<float*>(st + kC2NCSeg * 4);  // per-entry dcw (G > 1, MODE 1)
 
     // the item's table entries, 16 dwords each (one uint4 per thread), loaded
     // together with the grid below and stored after it
@@ -275,13 +308,16 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     const bool tlive = (uint32_t)tid < ne * 4;
     uint4 tv = make_uint4(0u, 0u, 0u, 0u);
     if (tlive) tv = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + tid];
+    constexpr bool kDcwE = G > 1 && MODE == 1;
+    float dj = 1.0f;
+    if (kDcwE && (uint32_t)tid < ne) dj = dcw_e[it.begin + tid];
 
     // tile + halo: a thread owns one float (cell ex, re or im) of a staged row
     // and walks the E1 rows for 4 coils; the row wrap and centring sign are
     // worked out once per row.  All E1 x 4 loads are issued before the first
     // LDS store, so the staging pays the memory latency once.
     {
-        constexpr int kU = kCoilsPerWave / 8;
+        constexpr int kU = CW / 8;
         const int q = tid & 31, cg = tid >> 5;  // 8 coil groups of 32 threads
         const bool live = q < 2 * E0;
         const int ex = live ? q >> 1 : 0, comp = q & 1;
@@ -302,36 +338,39 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
             }
         }
         if (live) {
-            float* dcol = sg + ex * kSubP + comp;
+            float* dcol = sg + ex * SP + comp;
 #pragma unroll
             for (int ey = 0; ey < E1; ey++) {
                 const int gy = sy0 + ey;  // parity of the wrapped row equals that of gy (even N_os)
                 const float sgn = (sign && ((gx + gy) & 1)) ? -1.0f : 1.0f;
 #pragma unroll
-                for (int u = 0; u < kU; u++) dcol[ey * E0 * kSubP + 2 * (cg + 8 * u)] = v[ey][u] * sgn;
+                for (int u = 0; u < kU; u++) dcol[ey * E0 * SP + 2 * (cg + 8 * u)] = v[ey][u] * sgn;
             }
         }
-        for (int i = tid; i < (E0 + 1) * kSubP; i += 64 * kWaves) sg[E0 * E1 * kSubP + i] = 0.0f;
+        for (int i = tid; i < (E0 + 1) * SP; i += 64 * kWaves) sg[E0 * E1 * SP + i] = 0.0f;
     }
     if (tlive) st[tid] = tv;
+    if (kDcwE && (uint32_t)tid < ne) sde[tid] = dj;
     __syncthreads();
 
-    // lanes past the last coil mirror coil c0 (its staged values, its output
-    // address): they store the same value to the same place, so no lane masking
-    const int coil = c0 + (lane >> 1);
+    // lane = (slot, coil, re/im); lanes past the last coil mirror coil c0 (its
+    // staged values, its output address): they store the same value to the
+    // same place, so no lane masking
+    const int slot = lane / (2 * CW), ql = lane % (2 * CW);
+    const int coil = c0 + (ql >> 1);
     const bool on = coil < C;
     const int lc = on ? coil : c0;
     const long long FM = (long long)F * M;
     const int dm = dcw_mode(dcw, dcw_len, FM);
-    float* ol = out + 2 * (long long)lc * FM + (lane & 1);
-    const float* sl = sg + (on ? lane : (lane & 1));
+    float* ol = out + 2 * (long long)lc * FM + (ql & 1);
+    const float* sl = sg + (on ? ql : (ql & 1));
     // wave w takes a contiguous quarter of the entries (consecutive samples of
-    // a spoke share output cache lines)
+    // a spoke share output cache lines), G at a time
     const uint32_t q0 = (ne * (uint32_t)wave) / kWaves, q1 = (ne * (uint32_t)(wave + 1)) / kWaves;
-    // window sum of entry k; j is its (wave-uniform) sample index
+    // window sum of entry k; j is its sample index (wave-uniform for G = 1)
     auto gather = [&](uint32_t k, uint32_t& j) -> float {
         const uint4 h0 = st[4 * k], h1 = st[4 * k + 1], h2 = st[4 * k + 2], h3 = st[4 * k + 3];
-        j = (uint32_t)__builtin_amdgcn_readfirstlane((int)h0.x);
+        j = G == 1 ? (uint32_t)__builtin_amdgcn_readfirstlane((int)h0.x) : h0.x;
         const uint32_t pk = h0.y;
         const int bx = (int)(int8_t)(pk & 0xff), by = (int)(int8_t)((pk >> 8) & 0xff);
         const float wx[kTabKW] = {__uint_as_float(h0.z), __uint_as_float(h0.w), __uint_as_float(h1.x),
@@ -342,12 +381,12 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
                                   __uint_as_float(h3.w)};
         // every slot of the window is read (slots outside it have weight 0
         // and stay inside the staged tile and its zeroed pad), all reads first
-        const float* base = sl + (by * E0 + bx) * kSubP;
+        const float* base = sl + (by * E0 + bx) * SP;
         float v[KW][KW];
 #pragma unroll
         for (int iy = 0; iy < KW; iy++)
 #pragma unroll
-            for (int ix = 0; ix < KW; ix++) v[iy][ix] = base[(iy * E0 + ix) * kSubP];
+            for (int ix = 0; ix < KW; ix++) v[iy][ix] = base[(iy * E0 + ix) * SP];
         float a = 0.0f;
 #pragma unroll
         for (int iy = 0; iy < KW; iy++) {
@@ -359,18 +398,20 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
         return a;
     };
     if constexpr (MODE < 2) {
-        // no vector loads in the loop (a per-sample dcw is a scalar load), so
-        // no wait ever covers the stores already issued
-        for (uint32_t k = q0; k < q1; k++) {
+        // no vector loads in the loop (a per-sample dcw is a scalar load, or
+        // staged), so no wait ever covers the stores already issued
+        for (uint32_t k = q0; k < q1; k += G) {
+            const uint32_t e = G == 1 ? k : min(k + (uint32_t)slot, q1 - 1);  // a short tail repeats the last entry
             uint32_t j;
-            float a = gather(k, j);
-            if constexpr (MODE == 1) a *= dcw[j];
+            float a = gather(e, j);
+            if constexpr (MODE == 1) a *= G == 1 ? dcw[j] : sde[e];
             ol[2 * (long long)j] = a;
         }
     } else {
-        for (uint32_t k = q0; k < q1; k++) {
+        for (uint32_t k = q0; k < q1; k += G) {
+            const uint32_t e = G == 1 ? k : min(k + (uint32_t)slot, q1 - 1);
             uint32_t j;
-            float a = gather(k, j);
+            float a = gather(e, j);
             if (dm == 1) a *= dcw[j];
             else if (dm == 2) a *= dcw[dcw_index(lc, j, FM, dcw_len)];
             const long long o = 2 * (long long)j;

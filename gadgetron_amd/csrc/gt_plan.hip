@@ -508,14 +508,25 @@ struct LaunchTablesC2NC {
     }
 };
 
+// Coil groups (NC2C) / sample slots (C2NC) of the 2D gridding kernels: a wave
+// covers 32 / G coils, so fewer coils leave no lanes idle.
+int coil_groups(int C) { return C <= 8 ? 4 : (C <= 16 ? 2 : 1); }
+
 template <int KW>
 struct LaunchFastNC2C {
+    template <int G>
+    static auto pick(const float* dcw, long long dcw_len, long long FM) {
+        return !dcw ? gt::nc2c_reg2d_kernel<0, G>
+                    : (dcw_len == FM ? gt::nc2c_reg2d_kernel<1, G> : gt::nc2c_reg2d_kernel<2, G>);
+    }
     static void run(const float* samples, float* grid, float* partials, const gt_nufft_plan* p, int C,
                     const gt::Layout& L, int sign, const float* dcw, long long dcw_len, int accumulate,
                     hipStream_t s) {
-        const unsigned chunks = (unsigned)((C + gt::kCoilsPerWave - 1) / gt::kCoilsPerWave);
+        const int G = coil_groups(C);
+        const int cw = gt::kCoilsPerWave / G;
+        const unsigned chunks = (unsigned)((C + cw - 1) / cw);
         const long long FM = (long long)(p->M * p->F);
-        auto k = !dcw ? gt::nc2c_reg2d_kernel<0> : (dcw_len == FM ? gt::nc2c_reg2d_kernel<1> : gt::nc2c_reg2d_kernel<2>);
+        auto k = G == 4 ? pick<4>(dcw, dcw_len, FM) : (G == 2 ? pick<2>(dcw, dcw_len, FM) : pick<1>(dcw, dcw_len, FM));
         hipLaunchKernelGGL(k, dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s,
                            samples, grid, partials, p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(),
                            p->bn.items.as<gt::WorkItem>(),
@@ -525,17 +536,29 @@ struct LaunchFastNC2C {
 
 template <int KW>
 struct LaunchFastC2NC {
-    static void run(const float* grid, float* out, const gt_nufft_plan* p, int C, const gt::Layout& L, int sign,
+    template <int G>
+    static auto pick(int mode) {
+        return mode == 0 ? gt::c2nc_multi2d_kernel<KW, 0, G>
+                         : (mode == 1 ? gt::c2nc_multi2d_kernel<KW, 1, G> : gt::c2nc_multi2d_kernel<KW, 2, G>);
+    }
+    static void run(const float* grid, float* out, gt_nufft_plan* p, int C, const gt::Layout& L, int sign,
                     const float* dcw, long long dcw_len, int accumulate, hipStream_t s) {
-        const gt::GridGeom& g = p->bc.geom;
-        const unsigned chunks = (unsigned)((C + gt::kCoilsPerWave - 1) / gt::kCoilsPerWave);
-        const size_t lds = size_t(gt::c2nc_lds_bytes(KW));
+        Binning& b = p->bc;
+        const int G = coil_groups(C);
+        const int cw = gt::kCoilsPerWave / G;
+        const unsigned chunks = (unsigned)((C + cw - 1) / cw);
+        const size_t lds = size_t(gt::c2nc_lds_bytes(KW, G));
         const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
-        auto k = mode == 0 ? gt::c2nc_multi2d_kernel<KW, 0>
-                           : (mode == 1 ? gt::c2nc_multi2d_kernel<KW, 1> : gt::c2nc_multi2d_kernel<KW, 2>);
-        hipLaunchKernelGGL(k, dim3((unsigned)p->bc.n_items, chunks), dim3(64 * gt::kWaves), lds, s,
-                           grid, out, p->bc.table.as<uint32_t>(), p->bc.items.as<gt::WorkItem>(), int(p->M),
-                           int(p->F), C, g, L, sign, dcw, dcw_len, accumulate);
+        if (mode == 1 && G > 1) {
+            // per-sample dcw in table order, staged with the table
+            b.dcw_e.reserve(std::max<size_t>(b.n, 1) * sizeof(float));
+            hipLaunchKernelGGL(gt::entry_dcw_kernel, dim3((unsigned)((b.n + 255) / 256)), dim3(256), 0, s,
+                               b.list.as<uint32_t>(), (long long)b.n, dcw, b.dcw_e.as<float>());
+        }
+        auto k = G == 4 ? pick<4>(mode) : (G == 2 ? pick<2>(mode) : pick<1>(mode));
+        hipLaunchKernelGGL(k, dim3((unsigned)b.n_items, chunks), dim3(64 * gt::kWaves), lds, s, grid, out,
+                           b.table.as<uint32_t>(), b.items.as<gt::WorkItem>(), int(p->M), int(p->F), C, b.geom, L,
+                           sign, dcw, dcw_len, b.dcw_e.as<float>(), accumulate);
     }
 };
 

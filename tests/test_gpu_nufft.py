@@ -116,6 +116,45 @@ def test_all_compute_modes_multiframe_multicoil(gpu, oversampled):
     assert rel_l2(dout.numpy(), ref) <= TOL
 
 
+@pytest.mark.parametrize("C,dcw_kind", [(1, "sample"), (6, "element"), (12, "sample"), (16, "element"),
+                                         (17, "sample"), (40, "element")])
+def test_coil_groups_2d(gpu, C, dcw_kind):
+    """2D gridding at coil counts that take 4, 2 and 1 coil groups per wave (and
+    a second coil chunk), two frames, per-sample and per-element dcw, and
+    convolve with accumulation."""
+    say(f"2D 64^2 coil groups: C={C}, dcw {dcw_kind}")
+    rng = np.random.default_rng(100 + C)
+    N = [64, 64]
+    traj = tj.golden_radial_2d(128, 60, frames=2)
+    F, M = traj.shape[0], traj.shape[1]
+    plan = gpu_plan(N)
+    plan.preprocess(dev(traj))
+    op = oracle_plan(N, traj)
+    smp = tj.complex_normal(rng, (C, F, M))
+    img = tj.complex_normal(rng, (C, F, 64, 64))
+    if dcw_kind == "sample":
+        dcw = tj.ramp_dcf(traj)
+    else:
+        dcw = rng.uniform(0.2, 1.0, C * F * M).astype(np.float32)
+    got = run(plan, smp, img.shape, CM.BACKWARDS_NC2C, dcw)
+    assert rel_l2(got, op.compute(smp, img.shape, po.BACKWARDS_NC2C, dcw=dcw)) <= TOL
+    got = run(plan, img, smp.shape, CM.FORWARDS_C2NC, dcw)
+    assert rel_l2(got, op.compute(img, smp.shape, po.FORWARDS_C2NC, dcw=dcw)) <= TOL
+    say(f"  C={C}: convolve with accumulation")
+    grid = tj.complex_normal(rng, (C, F, 96, 96))
+    acc = tj.complex_normal(rng, (C, F, M))
+    d_g, d_a = dev(grid, np.complex64), dev(acc, np.complex64)
+    plan.convolve(d_g, d_a, NFFT_conv_mode.C2NC, accumulate=True)
+    plan.synchronize()
+    ref = op.convolve(grid.reshape(-1), 0, C * F, out=acc.reshape(-1).copy(), accumulate=True)
+    assert rel_l2(d_a.numpy().reshape(-1), ref) <= TOL
+    d_s, d_g2 = dev(smp, np.complex64), dev(grid, np.complex64)
+    plan.convolve(d_s, d_g2, NFFT_conv_mode.NC2C, accumulate=True)
+    plan.synchronize()
+    ref = op.convolve(smp.reshape(-1), 1, C * F, out=grid.reshape(-1).copy(), accumulate=True)
+    assert rel_l2(d_g2.numpy().reshape(-1), ref) <= TOL
+
+
 def test_standalone_convolve_fft_deapodize(gpu):
     say("standalone convolve / fft / deapodize")
     rng = np.random.default_rng(21)
