@@ -154,15 +154,26 @@ __global__ __launch_bounds__(kRedThreads) void cg_init_finish_kernel(const doubl
     }
 }
 
-// q = w_E q + kappa p; partial sums of <p, q>.
-__global__ __launch_bounds__(kRedThreads) void cg_q_kernel(const float2* __restrict__ p, float2* __restrict__ q,
-                                                           CgParams P, double2* __restrict__ partial) {
+// q = w_E sum_c conj(csm_c) img_c + kappa p: the coil sum of E^H E p (in the
+// order of csm_conj_sum_kernel) fused with the CG step; partial sums of <p, q>.
+__global__ __launch_bounds__(kRedThreads) void cg_q_csm_kernel(const float2* __restrict__ img,
+                                                               const float2* __restrict__ csm, long long nimg, int F,
+                                                               int C, const float2* __restrict__ p,
+                                                               float2* __restrict__ q, CgParams P,
+                                                               double2* __restrict__ partial) {
     __shared__ double2 sh[kRedThreads];
     double2 acc = make_double2(0.0, 0.0);
     for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < P.n; i += (long long)gridDim.x * kRedThreads) {
+        const long long f = i / nimg, n = i - f * nimg;
+        float2 a = make_float2(0.0f, 0.0f);
+        for (int c = 0; c < C; c++) {
+            const float2 s = csm[n + nimg * c];
+            const float2 v = img[n + nimg * (f + (long long)F * c)];
+            a.x += s.x * v.x + s.y * v.y;
+            a.y += s.x * v.y - s.y * v.x;
+        }
         const float2 pv = p[i];
-        float2 qv = q[i];
-        qv = make_float2(P.we_re * qv.x + P.kappa * pv.x, P.we_re * qv.y + P.kappa * pv.y);
+        const float2 qv = make_float2(P.we_re * a.x + P.kappa * pv.x, P.we_re * a.y + P.kappa * pv.y);
         q[i] = qv;
         acc.x += (double)pv.x * qv.x + (double)pv.y * qv.y;   // Re(conj(p) q)
         acc.y += (double)pv.x * qv.y - (double)pv.y * qv.x;   // Im(conj(p) q)
@@ -374,15 +385,19 @@ struct gt_sense_op {
     }
 
     void mult_M(const void* in, void* out, bool accumulate, hipStream_t s) {
-        size_t di[5], ds[3] = {M, F, C};
-        int ni;
-        img_dims(di, ni, true);
-        dim3 grid(blocks_for((long long)nimg, 256), (unsigned)F);
-        hipLaunchKernelGGL(csm_mult_kernel, grid, dim3(256), 0, s, (const float2*)in, coil_img.as<float2>(), csm,
-                           (long long)nimg, int(F), int(C));
-        GT_HIP(hipGetLastError());
         void* dst = accumulate ? coil_smp.ptr : out;
-        check(gt_nufft_compute(plan, coil_img.ptr, di, ni, dst, ds, 3, dcw, dcw_len, GT_FORWARDS_C2NC, s));
+        // the plan multiplies the coil maps inside its first FFT pass where it
+        // has one; otherwise the coil images go through coil_img
+        if (!forward_with_coil_maps(plan, in, csm, C, dst, dcw, dcw_len, s)) {
+            size_t di[5], ds[3] = {M, F, C};
+            int ni;
+            img_dims(di, ni, true);
+            dim3 grid(blocks_for((long long)nimg, 256), (unsigned)F);
+            hipLaunchKernelGGL(csm_mult_kernel, grid, dim3(256), 0, s, (const float2*)in, coil_img.as<float2>(), csm,
+                               (long long)nimg, int(F), int(C));
+            GT_HIP(hipGetLastError());
+            check(gt_nufft_compute(plan, coil_img.ptr, di, ni, dst, ds, 3, dcw, dcw_len, GT_FORWARDS_C2NC, s));
+        }
         if (accumulate) {
             const long long n = (long long)(M * F * C);
             hipLaunchKernelGGL(add_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, (float2*)out,
@@ -402,13 +417,18 @@ struct gt_sense_op {
         GT_HIP(hipGetLastError());
     }
 
-    void mult_MH_M(const void* in, void* out, bool accumulate, hipStream_t s) {
+    // E^H E in, leaving the coil images E^H E before the coil sum in coil_img
+    void mult_MH_M_coils(const void* in, hipStream_t s) {
         mult_M(in, coil_smp.ptr, false, s);
         // coil_smp is consumed by NC2C before mult_MH writes coil_img
         size_t di[5], ds[3] = {M, F, C};
         int ni;
         img_dims(di, ni, true);
         check(gt_nufft_compute(plan, coil_smp.ptr, ds, 3, coil_img.ptr, di, ni, dcw, dcw_len, GT_BACKWARDS_NC2C, s));
+    }
+
+    void mult_MH_M(const void* in, void* out, bool accumulate, hipStream_t s) {
+        mult_MH_M_coils(in, s);
         dim3 grid(blocks_for((long long)nimg, 256), (unsigned)F);
         hipLaunchKernelGGL(csm_conj_sum_kernel, grid, dim3(256), 0, s, coil_img.as<float2>(), (float2*)out, csm,
                            (long long)nimg, int(F), int(C), accumulate ? 1 : 0);
@@ -484,9 +504,10 @@ void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, b
     if (!host_sync) {
         // device-side: iteration it reads state it and writes state it + 1
         for (int it = 0; it < d->iterations; it++) {
-            op->mult_MH_M(op->p.ptr, op->q.ptr, false, s);
-            hipLaunchKernelGGL(cg_q_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->p.as<float2>(),
-                               op->q.as<float2>(), P, part);
+            op->mult_MH_M_coils(op->p.ptr, s);
+            hipLaunchKernelGGL(cg_q_csm_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->coil_img.as<float2>(), op->csm,
+                               (long long)op->nimg, int(op->F), int(op->C), op->p.as<float2>(), op->q.as<float2>(), P,
+                               part);
             hipLaunchKernelGGL(cg_xr_step_kernel, dim3(nb), dim3(kRedThreads), 0, s, xf, op->r.as<float2>(),
                                op->p.as<float2>(), op->q.as<float2>(), P, st + it, part, int(nb), part + nb);
             const float2* z = P.w ? op->q.as<float2>() : op->r.as<float2>();
@@ -500,8 +521,9 @@ void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, b
     if (h.done) return;
     for (int it = 0; it < d->iterations; it++) {
         // q = (w_E E^H E + kappa I) p
-        op->mult_MH_M(op->p.ptr, op->q.ptr, false, s);
-        hipLaunchKernelGGL(cg_q_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->p.as<float2>(), op->q.as<float2>(), P,
+        op->mult_MH_M_coils(op->p.ptr, s);
+        hipLaunchKernelGGL(cg_q_csm_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->coil_img.as<float2>(), op->csm,
+                           (long long)op->nimg, int(op->F), int(op->C), op->p.as<float2>(), op->q.as<float2>(), P,
                            part);
         hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st);
         GT_HIP(hipGetLastError());

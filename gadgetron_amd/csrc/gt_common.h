@@ -87,4 +87,12 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Plan-internal entry for the SENSE operator (gt_cg.hip): FORWARDS_C2NC of the
+// image x [N..., F] weighted by the coil maps csm [N..., C] into samples
+// [M, F, C], with the coil multiply done inside the first FFT pass.  Returns
+// false, having done nothing, where the plan's compute has no such pass; the
+// caller then multiplies the coil maps itself.
+bool forward_with_coil_maps(gt_nufft_plan* p, const void* x, const void* csm, size_t C, void* out,
+                            const float* dcw, size_t dcw_len, hipStream_t s);
+
 }  // namespace gt_host

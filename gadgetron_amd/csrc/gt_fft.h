@@ -197,6 +197,10 @@ struct FftGeom {
     const float* fx;    // reciprocal deapodization filters (length Ox, Oy)
     const float* fy;
     float scale;        // 1/sqrt(G) times the centring global sign
+    // forwards only: coil maps [Ny][Nx] per coil (or null); batch b = c * F + f
+    // then reads image f times map c (the SENSE coil multiply)
+    const float2* csm;
+    int F;
 };
 
 constexpr int kFftRowThreads = 64;  // threads per row transform
@@ -220,7 +224,10 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_pad_kernel(
     const int b = live ? r / fg.Ny : 0, iy = live ? r % fg.Ny : 0;
     const int y = iy + fg.offy;
     const float sy = fg.scale * fg.fy[y] * ((y & 1) ? -1.0f : 1.0f);
-    const float2* in = img + ((long long)b * fg.Ny + iy) * fg.Nx;
+    const bool coils = fg.csm != nullptr;
+    const int fb = coils ? b % fg.F : b;  // image batch read
+    const float2* in = img + ((long long)fb * fg.Ny + iy) * fg.Nx;
+    const float2* cm = coils ? fg.csm + ((long long)(b / fg.F) * fg.Ny + iy) * fg.Nx : nullptr;
     float2* sb = &sbuf[rl][0];
     // every load of the row before the first LDS store (one memory latency)
     constexpr int kPer = (OX + kFftRowThreads - 1) / kFftRowThreads;
@@ -233,6 +240,16 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_pad_kernel(
         const bool ok = live && x < OX && ix >= 0 && ix < fg.Nx;
         v[i] = ok ? in[ix] : make_float2(0.0f, 0.0f);
         fxs[i] = x < OX ? fg.fx[x] : 0.0f;
+    }
+    if (coils) {
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int x = t + i * kFftRowThreads;
+            const int ix = x - fg.offx;
+            const bool ok = live && x < OX && ix >= 0 && ix < fg.Nx;
+            const float2 c = ok ? cm[ix] : make_float2(0.0f, 0.0f);
+            v[i] = make_float2(v[i].x * c.x - v[i].y * c.y, v[i].x * c.y + v[i].y * c.x);
+        }
     }
 #pragma unroll
     for (int i = 0; i < kPer; i++) {

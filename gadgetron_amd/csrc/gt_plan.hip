@@ -1160,8 +1160,10 @@ gt::FftGeom fft_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float sca
 // 2D hot path with the fused band transforms: pad + deapodize + row FFT over
 // the image rows, then column FFTs (C2NC side); column FFTs writing only the
 // cropped rows, then row FFT + crop + deapodize (NC2C side).
+// csm (FORWARDS_C2NC only): coil maps [N..., C]; `in` is then one image per
+// frame, multiplied by each coil's map inside the row pass.
 void compute_fast2d_fused(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const float* dcw,
-                          size_t dcw_len, int mode, float scale, hipStream_t s) {
+                          size_t dcw_len, int mode, float scale, hipStream_t s, const float2* csm = nullptr) {
     const size_t nb = sh.nb;
     const int Ox = int(p->Nos[0]), Oy = int(p->Nos[1]);
     GT_REQUIRE(nb * p->N[1] < (size_t(1) << 31) && nb * size_t((Ox + 15) / 16) < (size_t(1) << 31),
@@ -1173,7 +1175,9 @@ void compute_fast2d_fused(gt_nufft_plan* p, const void* in, void* out, const Sha
     const float2* twy = p->twiddle[1].as<float2>();
     if (mode == GT_FORWARDS_C2NC || mode == GT_BACKWARDS_C2NC) {
         const float dir = mode == GT_FORWARDS_C2NC ? -1.0f : 1.0f;
-        const gt::FftGeom fg = fft_geom(p, nb, true, scale);
+        gt::FftGeom fg = fft_geom(p, nb, true, scale);
+        fg.csm = csm;
+        fg.F = int(p->F);
         {
             StageTimer timer(p, GT_STAGE_FFT, s);
             dispatch_fft_len<LaunchRowsPad>(Ox, (const float2*)in, g, fg, twx, dir, s);
@@ -1299,6 +1303,22 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
 }
 
 }  // namespace
+
+bool gt_host::forward_with_coil_maps(gt_nufft_plan* p, const void* x, const void* csm, size_t C, void* out,
+                                     const float* dcw, size_t dcw_len, hipStream_t s) {
+    if (!(p->fast2d && p->fft_fused && fft2d_strategy() == 2)) return false;
+    GT_REQUIRE(x && csm && out && C > 0, GT_ERR_INVALID_ARGUMENT, "null argument");
+    GT_REQUIRE(p->M > 0 && p->F > 0, GT_ERR_NOT_PREPROCESSED, "plan has not been preprocessed");
+    GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
+    set_device(p);
+    Shape sh;
+    sh.oversampled = false;
+    sh.nb = p->F * C;
+    sh.C = C;
+    const float scale = centring_global_sign(p) / std::sqrt(float(p->G()));
+    compute_fast2d_fused(p, x, out, sh, dcw, dcw_len, GT_FORWARDS_C2NC, scale, s, static_cast<const float2*>(csm));
+    return true;
+}
 
 // ===========================================================================
 // C ABI
