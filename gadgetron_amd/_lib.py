@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = [
     "gt_sense_mult_MH",
     "gt_sense_mult_MH_M",
     "gt_cgsense_solve",
+    "gt_cgsense_solve_image",
     "gt_cgsense_batch_create",
     "gt_cgsense_batch_run",
     "gt_cgsense_batch_destroy",
@@ -151,6 +152,7 @@ def lib():
         "gt_sense_mult_MH": [P, P, P, I, P],
         "gt_sense_mult_MH_M": [P, P, P, I, P],
         "gt_cgsense_solve": [P, P, P, ctypes.POINTER(gt_cg_desc), ctypes.POINTER(D), P],
+        "gt_cgsense_solve_image": [P, P, P, ctypes.POINTER(gt_cg_desc), ctypes.POINTER(D), P],
         "gt_cgsense_batch_create": [PP, PP, PP, PP, I, ctypes.POINTER(gt_cg_desc), I, P],
         "gt_cgsense_batch_run": [P, P],
         "gt_cgsense_batch_destroy": [P],

@@ -115,15 +115,16 @@ __device__ __forceinline__ float2 precond2(const CgParams& P, long long i, float
     return make_float2(t.x * w.x - t.y * w.y, t.x * w.y + t.y * w.x);
 }
 
-// Initialisation: r = w_E * mhd; p = r (or P^2 r); partial sums of <r, p>.
+// Initialisation: r = rhs_scale * mhd; p = r (or P^2 r); partial sums of <r, p>.
 __global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __restrict__ mhd, float2* __restrict__ x,
                                                               float2* __restrict__ r, float2* __restrict__ p,
-                                                              CgParams P, double2* __restrict__ partial) {
+                                                              CgParams P, float rhs_scale,
+                                                              double2* __restrict__ partial) {
     __shared__ double2 sh[kRedThreads];
     double2 acc = make_double2(0.0, 0.0);
     for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < P.n; i += (long long)gridDim.x * kRedThreads) {
         const float2 d = mhd[i];
-        const float2 rv = make_float2(P.we_re * d.x, P.we_re * d.y);
+        const float2 rv = make_float2(rhs_scale * d.x, rhs_scale * d.y);
         const float2 pv = P.w ? precond2(P, i, rv) : rv;
         r[i] = rv;
         p[i] = pv;
@@ -485,7 +486,10 @@ void host_read(const gt_sense_op* op, CgState* h, hipStream_t s) {
 
 // The whole solve on one stream.  host_sync: read the scalars back after each
 // reduction (plain method); otherwise no host round trip (capturable).
-void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, bool host_sync, hipStream_t s) {
+// image_rhs: `data` is the right-hand side itself, an image [N..., F], instead
+// of samples whose w_E E^H image is the right-hand side.
+void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, bool host_sync, hipStream_t s,
+              bool image_rhs = false) {
     const CgParams P = cg_params(op, d);
     GT_REQUIRE(!P.w || (long long)op->image_elems() % P.wlen == 0, GT_ERR_SHAPE,
                "preconditioner length does not divide the image size");
@@ -496,9 +500,12 @@ void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, b
     float2* xf = static_cast<float2*>(x);
     CgState h{};
     // rhs = w_E E^H d (cgSolver::compute_rhs), x = 0, r = p = rhs (initialize)
-    op->mult_MH(data, op->mhd.ptr, false, s);
+    if (image_rhs)
+        GT_HIP(hipMemcpyAsync(op->mhd.ptr, data, op->image_elems() * sizeof(float2), hipMemcpyDeviceToDevice, s));
+    else
+        op->mult_MH(data, op->mhd.ptr, false, s);
     hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->mhd.as<float2>(), xf,
-                       op->r.as<float2>(), op->p.as<float2>(), P, part);
+                       op->r.as<float2>(), op->p.as<float2>(), P, image_rhs ? 1.0f : P.we_re, part);
     hipLaunchKernelGGL(cg_init_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st, hist);
     GT_HIP(hipGetLastError());
     if (!host_sync) {
@@ -726,6 +733,22 @@ gt_status gt_cgsense_solve(gt_sense_op* op, const void* data, void* x, const gt_
         hipStream_t s = S(stream);
         op->reserve_cg(desc->iterations);
         cg_solve(op, data, x, desc, false, s);
+        if (rq_history) {
+            GT_HIP(hipMemcpyAsync(rq_history, op->hist.ptr, size_t(desc->iterations + 1) * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+            GT_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
+gt_status gt_cgsense_solve_image(gt_sense_op* op, const void* rhs, void* x, const gt_cg_desc* desc,
+                                 double* rq_history, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(op && rhs && x, GT_ERR_INVALID_ARGUMENT, "null argument");
+        validate_desc(desc);
+        hipStream_t s = S(stream);
+        op->reserve_cg(desc->iterations);
+        cg_solve(op, rhs, x, desc, false, s, true);
         if (rq_history) {
             GT_HIP(hipMemcpyAsync(rq_history, op->hist.ptr, size_t(desc->iterations + 1) * sizeof(double),
                                   hipMemcpyDeviceToHost, s));

@@ -13,7 +13,7 @@ import numpy as np
 from ._lib import GTError, check, gt_cg_desc, lib
 from .nfft import NFFTPlan, _numel, _ptr, _stream
 
-__all__ = ["SenseOperator", "cg_sense", "CgBatch"]
+__all__ = ["SenseOperator", "cg_sense", "cg_sense_image", "CgBatch"]
 
 
 class SenseOperator:
@@ -73,6 +73,16 @@ def cg_sense(op: SenseOperator, data, x, iterations=10, tc_tolerance=1e-6, encod
     d = _desc(iterations, tc_tolerance, encoding_weight, kappa, precond)
     hist = (ctypes.c_double * (int(iterations) + 1))()
     check(lib().gt_cgsense_solve(op._h, _ptr(data), _ptr(x), ctypes.byref(d), hist, _stream(stream)))
+    return np.array(hist[:], dtype=np.float64)
+
+
+def cg_sense_image(op: SenseOperator, rhs, x, iterations=10, tc_tolerance=1e-6, encoding_weight=1.0, kappa=0.0,
+                   precond=None, stream=None):
+    """The solver of cg_sense with an image-space right-hand side:
+    x = CG(w_E E^H E + kappa I, rhs); returns the residual history."""
+    d = _desc(iterations, tc_tolerance, encoding_weight, kappa, precond)
+    hist = (ctypes.c_double * (int(iterations) + 1))()
+    check(lib().gt_cgsense_solve_image(op._h, _ptr(rhs), _ptr(x), ctypes.byref(d), hist, _stream(stream)))
     return np.array(hist[:], dtype=np.float64)
 
 

@@ -65,6 +65,14 @@ typedef struct gt_cg_desc {
 gt_status gt_cgsense_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* desc,
                            double* rq_history, gt_stream_t stream);
 
+/* The same solver with the right-hand side given as an image: x = CG(w_E E^H E
+ * + kappa I, rhs), rhs and x: [N..., F].  No counterpart in the reference: the
+ * backward pass of the differentiable solve needs it, since the gradient of
+ * x = A^-1 w_E E^H d with respect to d is w_E E A^-1 g (A is Hermitian), one
+ * more solve with the image-space gradient g as right-hand side. */
+gt_status gt_cgsense_solve_image(gt_sense_op* op, const void* rhs, void* x, const gt_cg_desc* desc,
+                                 double* rq_history, gt_stream_t stream);
+
 /* Many independent reconstructions (uneven sizes, one operator each), set up
  * once and run any number of times (the operators, data and x buffers are
  * bound at creation).
