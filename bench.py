@@ -227,6 +227,34 @@ def batched_cgsense(nprob=64, iterations=10, reps=3, seed=3):
     }
 
 
+def atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape, reps=10):
+    """The float-atomic NC2C (ConvolutionType::ATOMIC, ConvolverNC2C_atomic.cuh)
+    kept only as a measured comparison: its NC2C stage time against the
+    deterministic tile gather on the same samples, and how far its (run to run
+    varying) sums differ."""
+    from gadgetron_amd import DeviceArray, NFFT_comp_mode as CM
+    out = {}
+    imgs = {}
+    for name, conv in (("standard", ga.ConvolutionType.STANDARD), ("atomic", ga.ConvolutionType.ATOMIC)):
+        plan = ga.NFFTPlan(N, None, W=5.5, os_factor=1.5, conv=conv)
+        plan.preprocess(d_traj)
+        d_img = DeviceArray.zeros(img_shape, np.complex64)
+        plan.compute(d_s, d_img, d_w, CM.BACKWARDS_NC2C)
+        plan.synchronize()
+        plan.profile_read()
+        plan.profile(True)
+        for _ in range(reps):
+            plan.compute(d_s, d_img, d_w, CM.BACKWARDS_NC2C)
+        plan.synchronize()
+        plan.profile(False)
+        st = plan.profile_read()
+        out[f"{name}_nc2c_ms"] = round(st["nc2c"][0] / max(st["nc2c"][1], 1), 5)
+        imgs[name] = d_img.numpy()
+    a, b = imgs["standard"].ravel(), imgs["atomic"].ravel()
+    out["rel_diff"] = float(np.linalg.norm(a - b) / np.linalg.norm(a))
+    return out
+
+
 def load_traffic(config):
     path = os.path.join(ROOT, "profiles", f"traffic_{config}.json")
     if not os.path.exists(path):
@@ -341,6 +369,8 @@ def main():
         raise SystemExit("bench.py: non-finite output")
     ndft_err = ndft_error(d_img.numpy(), out, wl, rng)
 
+    atomic = atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape) if D == 2 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, samples, args.cpu_seconds)
@@ -370,6 +400,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched_cgsense": cgb,
+        "nc2c_atomic_comparison": atomic,
         "ndft_error": ndft_err,
         "stages_ms_per_step": {k: round(v[0] / args.steps, 5) for k, v in stages.items()},
         "preprocess_ms": round(prep_ms, 2),

@@ -51,6 +51,14 @@ EXPORTED_SYMBOLS = [
     "gt_cgsense_batch_run",
     "gt_cgsense_batch_destroy",
     "gt_cgsense_batch_branches",
+    # include/gt_radial.h
+    "gt_radial_trajectory_golden_ratio_2d",
+    "gt_radial_trajectory_fixed_angle_2d",
+    "gt_radial_trajectory_variable_angle_2d",
+    "gt_radial_dcw_golden_ratio_2d",
+    "gt_radial_dcw_fixed_angle_2d",
+    "gt_kooshball_trajectory_3d",
+    "gt_sdc_estimate_dcw",
 ]
 
 # gt_status
@@ -115,6 +123,7 @@ def lib():
             "(or __graft_entry__.build()).  There is no CPU fallback.")
     L = ctypes.CDLL(LIB_PATH)
     P, S, I, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+    U = ctypes.c_uint
     SP = ctypes.POINTER(ctypes.c_size_t)
     FP = ctypes.POINTER(ctypes.c_float)
     PP = ctypes.POINTER(ctypes.c_void_p)
@@ -157,6 +166,13 @@ def lib():
         "gt_cgsense_batch_run": [P, P],
         "gt_cgsense_batch_destroy": [P],
         "gt_cgsense_batch_branches": [P, ctypes.POINTER(I)],
+        "gt_radial_trajectory_golden_ratio_2d": [P, U, U, U, U, I, P],
+        "gt_radial_trajectory_fixed_angle_2d": [P, U, U, U, ctypes.c_float, P],
+        "gt_radial_trajectory_variable_angle_2d": [P, P, U, U, U, P],
+        "gt_radial_dcw_golden_ratio_2d": [P, U, U, ctypes.c_float, ctypes.c_float, U, I, P],
+        "gt_radial_dcw_fixed_angle_2d": [P, U, U, ctypes.c_float, ctypes.c_float, P],
+        "gt_kooshball_trajectory_3d": [P, U, U, P, P],
+        "gt_sdc_estimate_dcw": [P, S, I, SP, ctypes.c_float, U, ctypes.c_float, P, P, P],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
