@@ -255,6 +255,32 @@ def atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape, reps=10):
     return out
 
 
+def density_timing(ga, N, d_traj, samples_per_profile, profiles, reps=3):
+    """The step before the path, on the same trajectory: iterative SDC
+    (estimate_dcw: 10 Pipe-Menon iterations, Jinc kernel, os 2.1, width 5.5)
+    and the golden-ratio radial Voronoi weights."""
+    from gadgetron_amd import DeviceArray
+    from gadgetron_amd import radial as rd
+    n = samples_per_profile * profiles
+    out = DeviceArray.zeros((n,), np.float32)
+    rd.estimate_dcw(d_traj, N, 2.1, 10, 5.5, out=out)  # warm
+    ga.lib().gt_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rd.estimate_dcw(d_traj, N, 2.1, 10, 5.5, out=out)
+    ga.lib().gt_device_synchronize()
+    sdc_ms = (time.perf_counter() - t0) * 1e3 / reps
+    rd.dcw_golden_ratio_2d(samples_per_profile, profiles, 1.5, 1.0 / 1.5, out=out)
+    ga.lib().gt_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rd.dcw_golden_ratio_2d(samples_per_profile, profiles, 1.5, 1.0 / 1.5, out=out)
+    ga.lib().gt_device_synchronize()
+    vor_ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {"samples": n, "sdc_ms": round(sdc_ms, 3), "sdc_iterations": 10,
+            "radial_voronoi_dcw_ms": round(vor_ms, 3)}
+
+
 def load_traffic(config):
     path = os.path.join(ROOT, "profiles", f"traffic_{config}.json")
     if not os.path.exists(path):
@@ -370,6 +396,7 @@ def main():
     ndft_err = ndft_error(d_img.numpy(), out, wl, rng)
 
     atomic = atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape) if D == 2 else None
+    density = density_timing(ga, N, d_traj, 512, 400) if args.config == "c2" else None
 
     cpu = None
     if not args.no_cpu:
@@ -401,6 +428,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_cgsense": cgb,
         "nc2c_atomic_comparison": atomic,
+        "density_compensation": density,
         "ndft_error": ndft_err,
         "stages_ms_per_step": {k: round(v[0] / args.steps, 5) for k, v in stages.items()},
         "preprocess_ms": round(prep_ms, 2),

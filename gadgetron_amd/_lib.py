@@ -1,5 +1,5 @@
-"""ctypes binding of libgtnufft.so (the C ABI declared in include/gt_nufft.h
-and include/gt_cgsense.h).
+"""ctypes binding of libgtnufft.so (the C ABI declared in include/gt_nufft.h,
+gt_cgsense.h, gt_radial.h and gt_buffer.h).
 
 The library is built in-tree by gadgetron_amd/csrc/Makefile.  Loading fails
 loudly when it is missing: there is no CPU fallback for any entry point.
@@ -59,6 +59,16 @@ EXPORTED_SYMBOLS = [
     "gt_radial_dcw_fixed_angle_2d",
     "gt_kooshball_trajectory_3d",
     "gt_sdc_estimate_dcw",
+    # include/gt_buffer.h
+    "gt_buffer_create",
+    "gt_buffer_destroy",
+    "gt_buffer_setup",
+    "gt_buffer_set_dcw",
+    "gt_buffer_clear",
+    "gt_buffer_add_frame_data",
+    "gt_buffer_get_accumulated_coil_images",
+    "gt_buffer_get_normalization_factor",
+    "gt_buffer_get_combined_coil_image",
 ]
 
 # gt_status
@@ -173,13 +183,21 @@ def lib():
         "gt_radial_dcw_fixed_angle_2d": [P, U, U, ctypes.c_float, ctypes.c_float, P],
         "gt_kooshball_trajectory_3d": [P, U, U, P, P],
         "gt_sdc_estimate_dcw": [P, S, I, SP, ctypes.c_float, U, ctypes.c_float, P, P, P],
+        "gt_buffer_create": [PP],
+        "gt_buffer_destroy": [P],
+        "gt_buffer_setup": [P, I, SP, SP, ctypes.c_float, U, U, U],
+        "gt_buffer_set_dcw": [P, P, S],
+        "gt_buffer_clear": [P, P],
+        "gt_buffer_add_frame_data": [P, P, S, P, ctypes.POINTER(I), P],
+        "gt_buffer_get_accumulated_coil_images": [P, P, P],
+        "gt_buffer_get_normalization_factor": [P, FP],
+        "gt_buffer_get_combined_coil_image": [P, P, P, P],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ctypes.c_int
     L.gt_last_error.restype = ctypes.c_char_p
-    _ = FP
     _lib = L
     return _lib
 

@@ -643,7 +643,9 @@ std::unique_ptr<PlanBase> make_plan(int D, int precision, const size_t* N, const
         }
         p->kernel.compute_beta();
         // ctor 0 (hoNFFT(N, N_os, W)) builds filters with do_scale=true,
-        // ctor 1 (hoNFFT(N, alpha, W), the cuNFFT convention) with do_scale=false.
+        // ctor 1 (hoNFFT(N, alpha, W), the cuNFFT convention) with do_scale=false,
+        // ctor 2 (cuNFFT_plan(N, N_os, W): N_os given, the unscaled filter FFT of
+        // cuNFFT.cu:170-202) with do_scale=false as well.
         p->build_filters(ctor == 0);
     };
     if (precision == 0) {

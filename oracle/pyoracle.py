@@ -20,7 +20,7 @@ _lib = None
 
 FORWARDS_C2NC, FORWARDS_NC2C, BACKWARDS_C2NC, BACKWARDS_NC2C = 0, 1, 2, 3
 PREP_C2NC, PREP_NC2C, PREP_ALL = 0, 1, 2
-CTOR_NOS, CTOR_ALPHA = 0, 1
+CTOR_NOS, CTOR_ALPHA, CTOR_NOS_CU = 0, 1, 2
 
 
 def build():
@@ -86,9 +86,12 @@ def gt_dims(a):
 
 
 class Plan:
-    """hoNFFT_plan<REAL, D> restatement (hoNFFT.h:51-55 ctors)."""
+    """hoNFFT_plan<REAL, D> restatement (hoNFFT.h:51-55 ctors).  With Nos given,
+    cu=True selects cuNFFT_plan(N, N_os, W)'s deapodization scaling (unscaled
+    filter FFT, cuNFFT.cu:170-202) instead of hoNFFT_plan(N, N_os, W)'s
+    (scaled, hoNFFT.cpp:150-197): the outputs then differ by sqrt(prod N_os)."""
 
-    def __init__(self, N, W, Nos=None, alpha=None, precision="float"):
+    def __init__(self, N, W, Nos=None, alpha=None, precision="float", cu=False):
         self.D = len(N)
         self.precision = 0 if precision == "float" else 1
         self.cdt = np.complex64 if self.precision == 0 else np.complex128
@@ -96,7 +99,7 @@ class Plan:
         if alpha is not None:
             ctor, nos = CTOR_ALPHA, [0] * self.D
         else:
-            ctor, nos, alpha = CTOR_NOS, list(Nos), 0.0
+            ctor, nos, alpha = (CTOR_NOS_CU if cu else CTOR_NOS), list(Nos), 0.0
         h = lib().gto_plan_create(self.D, self.precision, _sizes(N), _sizes(nos), float(W), ctor, float(alpha))
         if not h:
             raise RuntimeError(lib().gto_last_error().decode())
