@@ -281,6 +281,39 @@ def density_timing(ga, N, d_traj, samples_per_profile, profiles, reps=3):
             "radial_voronoi_dcw_ms": round(vor_ms, 3)}
 
 
+def coil_map_timing(ga, N, d_traj, d_s, d_w, C, reps=3):
+    """The step before CG-SENSE on the same frame: the frame buffer
+    (add_frame_data = NC2C preprocess + accumulating gridding; readout =
+    backwards FFT, deapodization, crop) and the coil maps estimated from its
+    coil images (estimate_b1_map, 7x7 box, 2 power iterations)."""
+    from gadgetron_amd import DeviceArray
+    from gadgetron_amd.b1map import estimate_b1_map
+    from gadgetron_amd.buffer import Buffer
+    sync = ga.lib().gt_device_synchronize
+    buf = Buffer()
+    buf.setup(N, [int(n * 1.5) for n in N], 5.5, C, 4, 1)
+    buf.set_dcw(d_w)
+    imgs = DeviceArray.zeros((C,) + tuple(reversed(N)), np.complex64)
+    csm = DeviceArray.zeros((C,) + tuple(reversed(N)), np.complex64)
+
+    def timed(fn):
+        fn()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        sync()
+        return round((time.perf_counter() - t0) * 1e3 / reps, 3)
+
+    add_ms = timed(lambda: buf.add_frame_data(d_s, d_traj))
+    read_ms = timed(lambda: buf.get_accumulated_coil_images(out=imgs))
+    b1_ms = timed(lambda: estimate_b1_map(imgs, out=csm))
+    if not np.all(np.isfinite(csm.numpy())):
+        raise SystemExit("bench.py: non-finite coil maps")
+    return {"image": list(N), "coils": C, "buffer_add_frame_ms": add_ms, "buffer_readout_ms": read_ms,
+            "b1_map_ms": b1_ms}
+
+
 def load_traffic(config):
     path = os.path.join(ROOT, "profiles", f"traffic_{config}.json")
     if not os.path.exists(path):
@@ -397,6 +430,7 @@ def main():
 
     atomic = atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape) if D == 2 else None
     density = density_timing(ga, N, d_traj, 512, 400) if args.config == "c2" else None
+    coil_maps = coil_map_timing(ga, N, d_traj, d_s, d_w, C) if args.config == "c2" else None
 
     cpu = None
     if not args.no_cpu:
@@ -429,6 +463,7 @@ def main():
         "batched_cgsense": cgb,
         "nc2c_atomic_comparison": atomic,
         "density_compensation": density,
+        "coil_maps": coil_maps,
         "ndft_error": ndft_err,
         "stages_ms_per_step": {k: round(v[0] / args.steps, 5) for k, v in stages.items()},
         "preprocess_ms": round(prep_ms, 2),

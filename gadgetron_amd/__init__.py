@@ -15,10 +15,10 @@ from .nfft import (
     device_count,
 )
 from .sense import CgBatch, SenseOperator, cg_sense
-from . import buffer, trajectories
+from . import b1map, buffer, trajectories
 
 __all__ = [
     "CgBatch", "ConvolutionType", "DeviceArray", "EXPORTED_SYMBOLS", "GTError", "LIB_PATH", "NFFT_comp_mode",
     "NFFT_conv_mode", "NFFT_fft_mode", "NFFT_prep_mode", "NFFTPlan", "SenseOperator", "cg_sense",
-    "buffer", "device_count", "lib", "trajectories",
+    "b1map", "buffer", "device_count", "lib", "trajectories",
 ]

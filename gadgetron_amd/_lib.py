@@ -1,5 +1,5 @@
 """ctypes binding of libgtnufft.so (the C ABI declared in include/gt_nufft.h,
-gt_cgsense.h, gt_radial.h and gt_buffer.h).
+gt_cgsense.h, gt_radial.h, gt_buffer.h and gt_b1map.h).
 
 The library is built in-tree by gadgetron_amd/csrc/Makefile.  Loading fails
 loudly when it is missing: there is no CPU fallback for any entry point.
@@ -69,6 +69,8 @@ EXPORTED_SYMBOLS = [
     "gt_buffer_get_accumulated_coil_images",
     "gt_buffer_get_normalization_factor",
     "gt_buffer_get_combined_coil_image",
+    # include/gt_b1map.h
+    "gt_estimate_b1_map",
 ]
 
 # gt_status
@@ -192,6 +194,7 @@ def lib():
         "gt_buffer_get_accumulated_coil_images": [P, P, P],
         "gt_buffer_get_normalization_factor": [P, FP],
         "gt_buffer_get_combined_coil_image": [P, P, P, P],
+        "gt_estimate_b1_map": [P, I, SP, U, I, P, P],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

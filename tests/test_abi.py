@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     out = []
-    for h in ("gt_nufft.h", "gt_cgsense.h", "gt_radial.h", "gt_buffer.h"):
+    for h in ("gt_nufft.h", "gt_cgsense.h", "gt_radial.h", "gt_buffer.h", "gt_b1map.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         out += re.findall(r"^\s*(?:gt_status|const char\*)\s+(gt_\w+)\s*\(", text, flags=re.M)
     return out
