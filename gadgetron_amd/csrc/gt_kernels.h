@@ -105,6 +105,37 @@ __global__ void scale_traj_kernel(const float* __restrict__ traj, float* __restr
     out[i] = (k + 0.5f) * nos;
 }
 
+// SPARSE_MATRIX row normalisation (ConvolverNC2C_sparse.cuh:84-111): one over
+// the sum of a sample's window weights.  The weight of a cell is the product of
+// per-dimension weights, so the sum over the window is the product of the
+// per-dimension sums.
+__global__ void row_norm_kernel(const float* __restrict__ p, float* __restrict__ inv, long long n, GridGeom g) {
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float prod = 1.0f;
+    for (int d = 0; d < g.D; d++) {
+        const float x = p[j * g.D + d];
+        const int lo = (int)ceilf(x - g.radius), hi = (int)floorf(x + g.radius);
+        float s = 0.0f;
+        for (int c = lo; c <= hi; c++) {
+            const float u = fabsf((float)c - x);
+            if (!(u > g.radius)) s += kb1(u, g.nos_f[d], g.one_over_W, g.beta[d]);
+        }
+        prod *= s;
+    }
+    inv[j] = 1.0f / prod;
+}
+
+// w[c * FM + j] (or w[j] when the caller's dcw is per sample) = dcw * inv[j]:
+// the caller's density weights times the SPARSE_MATRIX row normalisation.
+__global__ void sparse_dcw_kernel(const float* __restrict__ dcw, long long dcw_len, const float* __restrict__ inv,
+                                  float* __restrict__ w, long long FM, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long j = i % FM;
+    w[i] = dcw[dcw_len == FM ? j : i % dcw_len] * inv[j];
+}
+
 // Distinct tiles touched by one sample's window along one dimension.
 __device__ __forceinline__ int tiles1d(float p, const GridGeom& g, int d, int (&t)[4]) {
     const int lo = (int)ceilf(p - g.radius);

@@ -159,6 +159,8 @@ struct gt_nufft_plan {
     size_t M = 0, F = 0;
     bool prep_c2nc = false, prep_nc2c = false;
     DevBuf traj;  // scaled positions [M*F][D]
+    DevBuf row_inv;     // SPARSE_MATRIX: 1 / window weight sum per sample [M*F]
+    DevBuf sparse_dcw;  // SPARSE_MATRIX: caller's dcw times row_inv
     DevBuf flag;
     Binning bn;   // NC2C: (tile, sample) pairs for every tile a window touches
     Binning bc;   // C2NC: one entry per sample in its home tile (2D hot path)
@@ -662,7 +664,7 @@ void setup_geometry(gt_nufft_plan* p) {
     // (the default; any other requested tile selects the generic kernels) and
     // a grid of at least tile + window cells per dimension.
     p->fast2d = false;
-    if (p->D == 2 && p->conv_type == GT_CONV_STANDARD && p->KW >= 3 && p->KW <= gt::kTabKW) {
+    if (p->D == 2 && p->conv_type != GT_CONV_ATOMIC && p->KW >= 3 && p->KW <= gt::kTabKW) {
         int ft[3];
         for (int d = 0; d < 3; d++) ft[d] = p->tile_req[d] > 0 ? p->tile_req[d] : gt::kRT;
         bool ok = true;
@@ -685,7 +687,7 @@ void setup_geometry(gt_nufft_plan* p) {
     // 3D hot path: STANDARD gridding, a window of 3..7 cells, no requested
     // tile, and a grid of at least tile + window cells per dimension.
     p->fast3d = false;
-    if (p->D == 3 && p->conv_type == GT_CONV_STANDARD && p->KW >= 3 && p->KW <= gt::kTabKW &&
+    if (p->D == 3 && p->conv_type != GT_CONV_ATOMIC && p->KW >= 3 && p->KW <= gt::kTabKW &&
         p->tile_req[0] <= 0 && p->tile_req[1] <= 0 && p->tile_req[2] <= 0) {
         const int tn[3] = {gt::k3T0, gt::k3T1, gt::k3T2}, tc[3] = {gt::k3S0, gt::k3S1, gt::k3S2};
         bool ok = true;
@@ -1018,6 +1020,23 @@ void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layo
                const float* dcw, size_t dcw_len, bool accumulate, hipStream_t s) {
     GT_REQUIRE(p->prep_nc2c, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for NC2C");
     Binning& b = p->bn;
+    if (p->conv_type == GT_CONV_SPARSE_MATRIX) {
+        // the row-normalised convolution matrix's transpose is the standard
+        // gridding of samples scaled by 1 / (window weight sum)
+        const long long FM = (long long)(p->M * p->F);
+        if (!dcw) {
+            dcw = p->row_inv.as<float>();
+            dcw_len = size_t(FM);
+        } else {
+            const long long n = dcw_len == size_t(FM) ? FM : FM * (long long)C;
+            p->sparse_dcw.reserve(size_t(n) * sizeof(float));
+            hipLaunchKernelGGL(gt::sparse_dcw_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dcw,
+                               (long long)dcw_len, p->row_inv.as<float>(), p->sparse_dcw.as<float>(), FM, n);
+            GT_HIP(hipGetLastError());
+            dcw = p->sparse_dcw.as<float>();
+            dcw_len = size_t(n);
+        }
+    }
     if (p->conv_type != GT_CONV_ATOMIC && b.n_partials > 0)
         p->partials.reserve(size_t(b.n_partials) * C * b.geom.tile_cells * sizeof(float2));
     StageTimer timer(p, GT_STAGE_NC2C, s);
@@ -1334,8 +1353,9 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         GT_REQUIRE(d->D >= 1 && d->D <= 3, GT_ERR_UNSUPPORTED, "D must be 1, 2 or 3");
         GT_REQUIRE(d->precision == GT_FLOAT, GT_ERR_UNSUPPORTED,
                    "only single precision (complext<float>) is provided");
-        GT_REQUIRE(d->conv_type == GT_CONV_STANDARD || d->conv_type == GT_CONV_ATOMIC, GT_ERR_UNSUPPORTED,
-                   "convolution type must be STANDARD or ATOMIC (SPARSE_MATRIX is not provided)");
+        GT_REQUIRE(d->conv_type == GT_CONV_STANDARD || d->conv_type == GT_CONV_ATOMIC ||
+                       d->conv_type == GT_CONV_SPARSE_MATRIX, GT_ERR_INVALID_ARGUMENT,
+                   "Invalid convolution type provided.");
         GT_REQUIRE(std::isfinite(d->W) && d->W >= 1.8, GT_ERR_INVALID_ARGUMENT,
                    "Kernel width must be equal to or larger than 1.8.");
         GT_REQUIRE(d->os_factor == 0.0 || d->os_factor >= 1.0, GT_ERR_INVALID_ARGUMENT,
@@ -1459,7 +1479,14 @@ gt_status gt_nufft_preprocess(gt_nufft_plan* p, const void* traj, size_t M, size
             p->prep_c2nc = true;
         }
         if (mode == GT_PREP_NC2C || mode == GT_PREP_ALL) {
-            if (p->conv_type == GT_CONV_STANDARD) build_nc2c(p, s);
+            if (p->conv_type != GT_CONV_ATOMIC) build_nc2c(p, s);
+            if (p->conv_type == GT_CONV_SPARSE_MATRIX) {
+                const long long FM = (long long)(M * F);
+                p->row_inv.reserve(size_t(FM) * sizeof(float));
+                hipLaunchKernelGGL(gt::row_norm_kernel, dim3((unsigned)((FM + 255) / 256)), dim3(256), 0, s,
+                                   p->traj.as<float>(), p->row_inv.as<float>(), FM, p->geom);
+                GT_HIP(hipGetLastError());
+            }
             p->prep_nc2c = true;
         }
     });

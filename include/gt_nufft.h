@@ -52,7 +52,10 @@ enum { GT_FORWARDS_C2NC = 0, GT_FORWARDS_NC2C = 1, GT_BACKWARDS_C2NC = 2, GT_BAC
 enum { GT_PREP_C2NC = 0, GT_PREP_NC2C = 1, GT_PREP_ALL = 2 };
 /* ConvolutionType (gpu/cuGriddingConvolution.h:13-18).  STANDARD is the
  * deterministic tile gather; ATOMIC is the float-atomic scatter kept for
- * comparison; SPARSE_MATRIX (row-normalised CSR) is not provided. */
+ * comparison; SPARSE_MATRIX grids through the transpose of the row-normalised
+ * convolution matrix (ConvolverNC2C_sparse.cuh:84-111): each sample is scaled
+ * by one over the sum of its window weights, then gridded as STANDARD (its
+ * C2NC is STANDARD's, as in the reference). */
 enum { GT_CONV_STANDARD = 0, GT_CONV_ATOMIC = 1, GT_CONV_SPARSE_MATRIX = 2 };
 /* precision */
 enum { GT_FLOAT = 0, GT_DOUBLE = 1 };
@@ -72,7 +75,7 @@ typedef struct gt_nufft_desc {
     size_t N_os[4];
     double os_factor;
     double W;            /* kernel width, >= 1.8 (cuGriddingConvolution.cu:67-69) */
-    int conv_type;       /* GT_CONV_STANDARD or GT_CONV_ATOMIC */
+    int conv_type;       /* GT_CONV_STANDARD, GT_CONV_ATOMIC or GT_CONV_SPARSE_MATRIX */
     int device;          /* -1: current device */
 } gt_nufft_desc;
 

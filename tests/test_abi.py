@@ -53,7 +53,7 @@ def _desc(**kw):
     (dict(D=4), 5),
     (dict(D=0), 5),
     (dict(precision=1), 5),                 # double not provided
-    (dict(conv_type=2), 5),                 # SPARSE_MATRIX not provided
+    (dict(conv_type=7), 1),                 # not a ConvolutionType
     (dict(N_os=[100, 192]), 1),             # N_os < N
     (dict(N_os=[193, 192]), 5),             # odd N_os
     (dict(os_factor=0.5), 1),               # oversampling factor < 1

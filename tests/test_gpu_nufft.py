@@ -318,3 +318,46 @@ def test_errors_on_device(gpu):
     with pytest.raises(ga.GTError) as e:
         plan.compute(z, y, None, CM.FORWARDS_C2NC)
     assert e.value.status == 3
+
+
+@pytest.mark.parametrize("N", [[64, 64], [80], [24, 24, 24]])
+def test_sparse_matrix_row_normalised(gpu, N):
+    """SPARSE_MATRIX: NC2C through the transpose of the row-normalised
+    convolution matrix = the oracle's gridding of samples / (sum of the
+    sample's window weights, from the oracle's own convolution rows); C2NC as
+    STANDARD."""
+    say(f"SPARSE_MATRIX {N}")
+    rng = np.random.default_rng(41)
+    D, M, C = len(N), 1500, 3
+    traj = rng.uniform(-0.5, 0.5, (1, M, D)).astype(np.float32)
+    plan = gpu_plan(N, conv=ga.ConvolutionType.SPARSE_MATRIX)
+    plan.preprocess(dev(traj))
+    op = oracle_plan(N, traj)
+    wsum = np.array([op.conv_row(j)[1].astype(np.float64).sum() for j in range(M)])
+    smp = tj.complex_normal(rng, (C, M))
+    gshape = (C,) + tuple(int(n * 1.5) for n in reversed(N))
+    d_g = DeviceArray.zeros(gshape, np.complex64)
+    plan.convolve(dev(smp), d_g, NFFT_conv_mode.NC2C)
+    plan.synchronize()
+    ref = op.convolve((smp / wsum).astype(np.complex64).reshape(-1), 1, C)
+    assert rel_l2(d_g.numpy().reshape(-1), ref) <= TOL
+    grid = tj.complex_normal(rng, gshape)
+    d_g = dev(grid)
+    plan.convolve(dev(smp), d_g, NFFT_conv_mode.NC2C, accumulate=True)
+    plan.synchronize()
+    assert rel_l2(d_g.numpy().reshape(-1), grid.reshape(-1) + ref) <= TOL
+    img_shape = (C,) + tuple(reversed(N))
+    w_s = rng.uniform(0.2, 1.0, M).astype(np.float32)
+    w_c = rng.uniform(0.2, 1.0, C * M).astype(np.float32)
+    for name, dcw, odcw in (("none", None, 1.0 / wsum), ("per sample", w_s, w_s / wsum),
+                            ("per coil", w_c, w_c / np.tile(wsum, C))):
+        got = run(plan, smp, img_shape, CM.BACKWARDS_NC2C, dcw)
+        want = op.compute(smp, img_shape, CM.BACKWARDS_NC2C, odcw.astype(np.float32))
+        assert rel_l2(got.reshape(-1), want.reshape(-1)) <= TOL, name
+    img = tj.complex_normal(rng, img_shape)
+    got = run(plan, img, (C, M), CM.FORWARDS_C2NC, w_s)
+    want = op.compute(img, (C, M), CM.FORWARDS_C2NC, w_s)
+    assert rel_l2(got.reshape(-1), want.reshape(-1)) <= TOL
+    # repeatable
+    a = run(plan, smp, img_shape, CM.BACKWARDS_NC2C, w_c)
+    assert np.array_equal(a, run(plan, smp, img_shape, CM.BACKWARDS_NC2C, w_c))
