@@ -67,8 +67,9 @@ gt_status gt_kooshball_trajectory_3d(float* traj, unsigned samples_per_spoke, un
  * kernel_width, on a grid of matrix_size * os_factor (truncated) rounded up to
  * multiples of 32 per dimension (the reference rounds to its GPUs' warp size).
  * traj: device n * D floats in [-1/2, 1/2] (one flattened frame); dcw: device n
- * floats (written).  The reference's defaults: os_factor 2.1, 10 iterations,
- * width 5.5.  Synchronises `stream`. */
+ * floats (written).  The reference's defaults: os_factor 2.1 (SDC.h:22-27;
+ * 1.5 in the cuNDArray overload, cuSDC.h:20-24), 10 iterations, width 5.5.
+ * Synchronises `stream`. */
 gt_status gt_sdc_estimate_dcw(const float* traj, size_t n, int D, const size_t* matrix_size, float os_factor,
                               unsigned iterations, float kernel_width, const float* initial_dcw, float* dcw,
                               gt_stream_t stream);
