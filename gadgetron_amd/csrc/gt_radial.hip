@@ -284,40 +284,103 @@ __device__ __forceinline__ float sdc_dist(const float* p, const int* c, const Sd
     return sqrtf(s);
 }
 
-// NC2C: one 64-thread block per tile, one tile cell per thread; the tile's
-// samples are walked in ascending order (bitwise repeatable).
-__global__ __launch_bounds__(64) void sdc_nc2c_kernel(const float* __restrict__ traj, const float* __restrict__ w,
-                                                      const uint32_t* __restrict__ list,
-                                                      const uint32_t* __restrict__ first,
-                                                      const uint32_t* __restrict__ last, SdcGeom g,
-                                                      float* __restrict__ grid) {
-    const int tile = blockIdx.x;
-    int org[kSdcMaxD] = {0, 0, 0}, tt = tile;
+// NC2C, split for radial sampling: the tiles at the centre of k-space hold far
+// more sample-tile pairs than the rest (every spoke crosses them), so a tile's
+// pairs are cut into segments of kSdcSeg (a compact list: seg_off[t] is tile
+// t's first segment), one 64-thread block per segment, one tile cell per
+// thread.  A block stages kSdcStage pairs at a time in LDS (position and
+// weight, one coalesced load per thread) and every thread walks them in
+// ascending order; sdc_reduce_kernel adds a tile's segment sums in segment
+// order, so the result is bitwise repeatable.
+constexpr uint32_t kSdcSeg = 512;
+constexpr int kSdcStage = 64;
+
+__device__ __forceinline__ void sdc_tile_origin(int tile, const SdcGeom& g, int* org) {
+    int tt = tile;
     for (int d = 0; d < g.D; d++) {
         org[d] = (tt % g.ntiles[d]) * g.tile[d];
         tt /= g.ntiles[d];
     }
+}
+
+// Segments per tile (at least one, so every tile writes its cells).
+__global__ void sdc_nseg_kernel(const uint32_t* __restrict__ first, const uint32_t* __restrict__ last, uint32_t n,
+                                uint32_t* __restrict__ nseg) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) nseg[t] = max(1u, (last[t] - first[t] + kSdcSeg - 1) / kSdcSeg);
+}
+
+__global__ void sdc_seg_map_kernel(const uint32_t* __restrict__ nseg, const uint32_t* __restrict__ seg_off,
+                                   uint32_t n, uint32_t* __restrict__ seg_tile) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    for (uint32_t k = 0; k < nseg[t]; k++) seg_tile[seg_off[t] + k] = t;
+}
+
+__global__ __launch_bounds__(64) void sdc_nc2c_kernel(const float* __restrict__ traj, const float* __restrict__ w,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ first,
+                                                      const uint32_t* __restrict__ last,
+                                                      const uint32_t* __restrict__ seg_tile,
+                                                      const uint32_t* __restrict__ seg_off, SdcGeom g,
+                                                      float* __restrict__ partial) {
+    __shared__ float4 sp[kSdcStage];
+    const uint32_t sg = blockIdx.x;
+    const int tile = (int)seg_tile[sg];
+    const uint32_t k = sg - seg_off[tile];
+    int org[kSdcMaxD] = {0, 0, 0};
+    sdc_tile_origin(tile, g, org);
     int c[kSdcMaxD] = {0, 0, 0}, lc = threadIdx.x;
-    bool inside = threadIdx.x < g.tile_cells;
     for (int d = 0; d < g.D; d++) {
         c[d] = org[d] + lc % g.tile[d];
         lc /= g.tile[d];
-        inside &= c[d] < g.nos[d];
     }
-    const uint32_t b = first[tile], e = last[tile];
+    const uint32_t b = first[tile] + k * kSdcSeg;
+    const uint32_t e = min(last[tile], b + kSdcSeg);
     float acc = 0.0f;
-    for (uint32_t i = b; i < e; i++) {
-        const uint32_t j = list[i];
-        float p[kSdcMaxD];
-        sdc_pos(traj, j, g, p);
-        acc += jinc_weight(sdc_dist(p, c, g), g.r0) * w[j];
+    for (uint32_t base = b; base < e; base += kSdcStage) {
+        const uint32_t i = base + threadIdx.x;
+        __syncthreads();
+        if (i < e) {
+            const uint32_t j = list[i];
+            float p[kSdcMaxD] = {0.0f, 0.0f, 0.0f};
+            sdc_pos(traj, j, g, p);
+            sp[threadIdx.x] = make_float4(p[0], p[1], p[2], w[j]);
+        }
+        __syncthreads();
+        const int n = (int)min((uint32_t)kSdcStage, e - base);
+        for (int q = 0; q < n; q++) {
+            const float4 v = sp[q];
+            const float pq[kSdcMaxD] = {v.x, v.y, v.z};
+            acc += jinc_weight(sdc_dist(pq, c, g), g.r0) * v.w;
+        }
     }
-    if (!inside) return;
+    partial[(long long)sg * 64 + threadIdx.x] = acc;
+}
+
+// grid cell = sum of its tile's segment sums, in segment order.
+__global__ __launch_bounds__(64) void sdc_reduce_kernel(const float* __restrict__ partial,
+                                                        const uint32_t* __restrict__ nseg,
+                                                        const uint32_t* __restrict__ seg_off, SdcGeom g,
+                                                        float* __restrict__ grid) {
+    const int tile = blockIdx.x;
+    int org[kSdcMaxD] = {0, 0, 0};
+    sdc_tile_origin(tile, g, org);
+    int lc = threadIdx.x;
+    bool inside = threadIdx.x < g.tile_cells;
     long long cell = 0, st = 1;
     for (int d = 0; d < g.D; d++) {
-        cell += (long long)c[d] * st;
+        const int cd = org[d] + lc % g.tile[d];
+        lc /= g.tile[d];
+        inside &= cd < g.nos[d];
+        cell += (long long)cd * st;
         st *= g.nos[d];
     }
+    if (!inside) return;
+    const float* src = partial + (long long)seg_off[tile] * 64 + threadIdx.x;
+    const uint32_t n = nseg[tile];
+    float acc = 0.0f;
+    for (uint32_t q = 0; q < n; q++) acc += src[(long long)q * 64];
     grid[cell] = acc;
 }
 
@@ -544,10 +607,32 @@ gt_status gt_sdc_estimate_dcw(const float* traj, size_t n, int D, const size_t* 
                            first.as<uint32_t>(), last.as<uint32_t>());
         const size_t G = size_t(g.nos[0]) * g.nos[1] * g.nos[2];
         grid.reserve(G * sizeof(float));
+        // the compact segment list
+        DevBuf nseg, seg_off, seg_tile, partial;
+        nseg.reserve(size_t(ntiles) * 4);
+        seg_off.reserve(size_t(ntiles) * 4);
+        hipLaunchKernelGGL(sdc_nseg_kernel, dim3(blocks(ntiles)), dim3(256), 0, s, first.as<uint32_t>(),
+                           last.as<uint32_t>(), ntiles, nseg.as<uint32_t>());
+        tb = 0;
+        GT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, nseg.as<uint32_t>(), seg_off.as<uint32_t>(), (int)ntiles, s));
+        temp.reserve(std::max<size_t>(tb, 1));
+        GT_HIP(hipcub::DeviceScan::ExclusiveSum(temp.ptr, tb, nseg.as<uint32_t>(), seg_off.as<uint32_t>(), (int)ntiles, s));
+        uint32_t so = 0, sn = 0;
+        GT_HIP(hipMemcpyAsync(&so, seg_off.as<uint32_t>() + ntiles - 1, 4, hipMemcpyDeviceToHost, s));
+        GT_HIP(hipMemcpyAsync(&sn, nseg.as<uint32_t>() + ntiles - 1, 4, hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        const uint32_t nsegs = so + sn;
+        seg_tile.reserve(size_t(nsegs) * 4);
+        partial.reserve(size_t(nsegs) * 64 * sizeof(float));
+        hipLaunchKernelGGL(sdc_seg_map_kernel, dim3(blocks(ntiles)), dim3(256), 0, s, nseg.as<uint32_t>(),
+                           seg_off.as<uint32_t>(), ntiles, seg_tile.as<uint32_t>());
         hipLaunchKernelGGL(sdc_fill_kernel, dim3(blocks(N)), dim3(256), 0, s, dcw, N, initial_dcw);
         for (unsigned it = 0; it < iterations; it++) {
-            hipLaunchKernelGGL(sdc_nc2c_kernel, dim3(ntiles), dim3(64), 0, s, traj, dcw, list.as<uint32_t>(),
-                               first.as<uint32_t>(), last.as<uint32_t>(), g, grid.as<float>());
+            hipLaunchKernelGGL(sdc_nc2c_kernel, dim3(nsegs), dim3(64), 0, s, traj, dcw, list.as<uint32_t>(),
+                               first.as<uint32_t>(), last.as<uint32_t>(), seg_tile.as<uint32_t>(),
+                               seg_off.as<uint32_t>(), g, partial.as<float>());
+            hipLaunchKernelGGL(sdc_reduce_kernel, dim3(ntiles), dim3(64), 0, s, partial.as<float>(),
+                               nseg.as<uint32_t>(), seg_off.as<uint32_t>(), g, grid.as<float>());
             hipLaunchKernelGGL(sdc_c2nc_update_kernel, dim3(blocks(N)), dim3(256), 0, s, traj, N, g, grid.as<float>(),
                                dcw);
         }
