@@ -209,7 +209,12 @@ __device__ __forceinline__ int dcw_mode(const float* dcw, long long dcw_len, lon
 // Rounded up to whole float4s: the work item's table entries follow it.
 // A staged cell holds the 2 x 32 / G floats of a wave's coils plus one (an
 // odd stride keeps rows conflict-free).
-__host__ __device__ constexpr int c2nc_cell_stride(int G) { return 2 * (kCoilsPerWave / G) + 1; }
+// LDS floats per staged cell of the 2D C2NC: 32 / G coils x (re, im) plus a
+// pad float; with 32 coils (G = 1) a lane reads a coil's (re, im) as one
+// 8-byte word, so the stride stays even.
+__host__ __device__ constexpr int c2nc_cell_stride(int G) {
+    return G == 1 ? 2 * kCoilsPerWave + 2 : 2 * (kCoilsPerWave / G) + 1;
+}
 
 __host__ __device__ inline int c2nc_lds_floats(int T0, int T1, int KW, int G = 1) {
     return (((T1 + KW) * (T0 + KW - 1) + 1) * c2nc_cell_stride(G) + 3) & ~3;

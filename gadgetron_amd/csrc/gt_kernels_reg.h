@@ -16,7 +16,8 @@
 //   A workgroup of kWaves wavefronts stages one tile plus the kernel halo
 //   (32 coils, centring sign applied) and the work item's table in LDS; each
 //   wavefront then takes a contiguous run of the tile's home samples and
-//   gathers each window with all of its reads in flight before the FMAs.
+//   gathers each window with all of its reads in flight before the FMAs
+//   (two samples per step: with 32 coils a lane holds one coil's (re, im)).
 #pragma once
 
 #include "gt_kernels_fast.h"
@@ -273,9 +274,10 @@ __global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restr
 
 // MODE 0: no dcw, no accumulation; 1: one dcw weight per sample, no
 // accumulation; 2: anything else (per-element dcw, accumulation).
-// G: samples gathered at a time per wave (1, 2 or 4) for 32 / G coils; lane =
-// (sample slot, coil, re/im).  With G > 1 a per-sample dcw comes from dcw_e
-// (per table entry), staged with the table.
+// G: coil groups (1, 2 or 4) for 32 / G coils.  G > 1: lane = (sample slot,
+// coil, re/im), G samples per step; G = 1: lane = (sample slot, coil) reading
+// (re, im) as one 8-byte word, 2 samples per step.  A per-sample dcw comes
+// from dcw_e (per table entry), staged with the table.
 // LDS holds at most 3 workgroups per CU (3 waves per SIMD): the waves-per-EU
 // range lets the scheduler keep a whole window's reads, and the staging's
 // loads, in flight.
@@ -308,7 +310,9 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     const bool tlive = (uint32_t)tid < ne * 4;
     uint4 tv = make_uint4(0u, 0u, 0u, 0u);
     if (tlive) tv = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + tid];
-    constexpr bool kDcwE = G > 1 && MODE == 1;
+    // a per-sample dcw is staged per entry (lanes of a wave serve several
+    // samples at once in every mode)
+    constexpr bool kDcwE = MODE == 1;
     float dj = 1.0f;
     if (kDcwE && (uint32_t)tid < ne) dj = dcw_e[it.begin + tid];
 
@@ -352,6 +356,76 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     if (tlive) st[tid] = tv;
     if (kDcwE && (uint32_t)tid < ne) sde[tid] = dj;
     __syncthreads();
+
+    if constexpr (G == 1) {
+        // 32 coils: lane = (slot of 2, coil) reads a coil's (re, im) as one
+        // 8-byte word, so a wave gathers two samples per step with half the LDS
+        // instructions of one lane per component; lanes past the last coil
+        // mirror coil c0 (same values, same output address)
+        const int slot = lane >> 5, coil = c0 + (lane & 31);
+        const bool on = coil < C;
+        const int lc = on ? coil : c0;
+        const long long FM = (long long)F * M;
+        const int dm = dcw_mode(dcw, dcw_len, FM);
+        float2* ol = reinterpret_cast<float2*>(out) + (long long)lc * FM;
+        constexpr int SP2 = SP / 2;
+        const float2* sl = reinterpret_cast<const float2*>(sg) + (on ? (lane & 31) : 0);
+        const uint32_t q0 = (ne * (uint32_t)wave) / kWaves, q1 = (ne * (uint32_t)(wave + 1)) / kWaves;
+        auto gather2 = [&](uint32_t k, uint32_t& j) -> float2 {
+            const uint4 h0 = st[4 * k], h1 = st[4 * k + 1], h2 = st[4 * k + 2], h3 = st[4 * k + 3];
+            j = h0.x;
+            const uint32_t pk = h0.y;
+            const int bx = (int)(int8_t)(pk & 0xff), by = (int)(int8_t)((pk >> 8) & 0xff);
+            const float wx[kTabKW] = {__uint_as_float(h0.z), __uint_as_float(h0.w), __uint_as_float(h1.x),
+                                      __uint_as_float(h1.y), __uint_as_float(h1.z), __uint_as_float(h1.w),
+                                      __uint_as_float(h2.x)};
+            const float wy[kTabKW] = {__uint_as_float(h2.y), __uint_as_float(h2.z), __uint_as_float(h2.w),
+                                      __uint_as_float(h3.x), __uint_as_float(h3.y), __uint_as_float(h3.z),
+                                      __uint_as_float(h3.w)};
+            const float2* base = sl + (by * E0 + bx) * SP2;
+            float2 v[KW][KW];
+#pragma unroll
+            for (int iy = 0; iy < KW; iy++)
+#pragma unroll
+                for (int ix = 0; ix < KW; ix++) v[iy][ix] = base[(iy * E0 + ix) * SP2];
+            float2 a = make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int iy = 0; iy < KW; iy++) {
+                float rx = 0.0f, ry = 0.0f;
+#pragma unroll
+                for (int ix = 0; ix < KW; ix++) {
+                    rx += wx[ix] * v[iy][ix].x;
+                    ry += wx[ix] * v[iy][ix].y;
+                }
+                a.x += wy[iy] * rx;
+                a.y += wy[iy] * ry;
+            }
+            return a;
+        };
+        for (uint32_t k = q0; k < q1; k += 2) {
+            const uint32_t e = min(k + (uint32_t)slot, q1 - 1);  // a short tail repeats the last entry
+            uint32_t j;
+            float2 a = gather2(e, j);
+            if constexpr (MODE == 1) {
+                const float d = sde[e];
+                a.x *= d;
+                a.y *= d;
+            } else if constexpr (MODE == 2) {
+                float d = 1.0f;
+                if (dm == 1) d = dcw[j];
+                else if (dm == 2) d = dcw[dcw_index(lc, j, FM, dcw_len)];
+                a.x *= d;
+                a.y *= d;
+                if (accumulate) {
+                    const float2 o = ol[j];
+                    a.x += o.x;
+                    a.y += o.y;
+                }
+            }
+            ol[j] = a;
+        }
+        return;
+    }
 
     // lane = (slot, coil, re/im); lanes past the last coil mirror coil c0 (its
     // staged values, its output address): they store the same value to the

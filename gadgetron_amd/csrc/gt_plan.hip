@@ -551,7 +551,7 @@ struct LaunchFastC2NC {
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
         const size_t lds = size_t(gt::c2nc_lds_bytes(KW, G));
         const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
-        if (mode == 1 && G > 1) {
+        if (mode == 1) {
             // per-sample dcw in table order, staged with the table
             b.dcw_e.reserve(std::max<size_t>(b.n, 1) * sizeof(float));
             hipLaunchKernelGGL(gt::entry_dcw_kernel, dim3((unsigned)((b.n + 255) / 256)), dim3(256), 0, s,
