@@ -221,7 +221,7 @@ __host__ __device__ inline int c2nc_lds_floats(int T0, int T1, int KW, int G = 1
 }
 
 // C2NC work items hold at most this many entries (their table is staged in LDS).
-constexpr int kC2NCSeg = 64;
+constexpr int kC2NCSeg = 128;
 
 __host__ __device__ inline int c2nc_lds_bytes(int KW, int G = 1) {
     return (c2nc_lds_floats(8, 8, KW, G) + kC2NCSeg * kTabDw + kC2NCSeg) * 4;

@@ -304,12 +304,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     uint4* st = reinterpret_cast<uint4*>(sg + c2nc_lds_floats(kRT, kRT, KW, G));
     float* sde = reinterpret_cast<float*>(st + kC2NCSeg * 4);  // per-entry dcw (G > 1, MODE 1)
 
-    // the item's table entries, 16 dwords each (one uint4 per thread), loaded
+    // the item's table entries, 16 dwords each (two uint4 per thread), loaded
     // together with the grid below and stored after it
-    static_assert(kC2NCSeg * 4 <= 64 * kWaves, "one table uint4 per thread");
-    const bool tlive = (uint32_t)tid < ne * 4;
-    uint4 tv = make_uint4(0u, 0u, 0u, 0u);
+    static_assert(kC2NCSeg * 4 <= 2 * 64 * kWaves, "two table uint4 per thread");
+    constexpr int kTT = 64 * kWaves;
+    const bool tlive = (uint32_t)tid < ne * 4, tlive2 = (uint32_t)(tid + kTT) < ne * 4;
+    uint4 tv = make_uint4(0u, 0u, 0u, 0u), tv2 = make_uint4(0u, 0u, 0u, 0u);
     if (tlive) tv = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + tid];
+    if (tlive2) tv2 = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + tid + kTT];
     // a per-sample dcw is staged per entry (lanes of a wave serve several
     // samples at once in every mode)
     constexpr bool kDcwE = MODE == 1;
@@ -354,6 +356,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
         for (int i = tid; i < (E0 + 1) * SP; i += 64 * kWaves) sg[E0 * E1 * SP + i] = 0.0f;
     }
     if (tlive) st[tid] = tv;
+    if (tlive2) st[tid + kTT] = tv2;
     if (kDcwE && (uint32_t)tid < ne) sde[tid] = dj;
     __syncthreads();
 
