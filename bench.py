@@ -331,6 +331,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the batched CG-SENSE comparison")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the atomic NC2C, density compensation and coil-map timings (profiling runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
     ap.add_argument("--max-items", type=int, default=0,
@@ -428,9 +430,10 @@ def main():
         raise SystemExit("bench.py: non-finite output")
     ndft_err = ndft_error(d_img.numpy(), out, wl, rng)
 
-    atomic = atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape) if D == 2 else None
-    density = density_timing(ga, N, d_traj, 512, 400) if args.config == "c2" else None
-    coil_maps = coil_map_timing(ga, N, d_traj, d_s, d_w, C) if args.config == "c2" else None
+    extras = not args.no_extras
+    atomic = atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape) if D == 2 and extras else None
+    density = density_timing(ga, N, d_traj, 512, 400) if args.config == "c2" and extras else None
+    coil_maps = coil_map_timing(ga, N, d_traj, d_s, d_w, C) if args.config == "c2" and extras else None
 
     cpu = None
     if not args.no_cpu:
