@@ -276,8 +276,9 @@ __global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restr
 // accumulation; 2: anything else (per-element dcw, accumulation).
 // G: coil groups (1, 2 or 4) for 32 / G coils.  G > 1: lane = (sample slot,
 // coil, re/im), G samples per step; G = 1: lane = (sample slot, coil) reading
-// (re, im) as one 8-byte word, 2 samples per step.  A per-sample dcw comes
-// from dcw_e (per table entry), staged with the table.
+// (re, im) as one 8-byte word, 2 samples per step.  A per-sample dcw is
+// loaded for each table entry while the tile is staged, and kept beside the
+// table (dcw_e is unused here).
 // LDS holds at most 3 workgroups per CU (3 waves per SIMD): the waves-per-EU
 // range lets the scheduler keep a whole window's reads, and the staging's
 // loads, in flight.
@@ -313,10 +314,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     if (tlive) tv = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + tid];
     if (tlive2) tv2 = reinterpret_cast<const uint4*>(tab)[(size_t)it.begin * 4 + tid + kTT];
     // a per-sample dcw is staged per entry (lanes of a wave serve several
-    // samples at once in every mode)
+    // samples at once in every mode): the thread holding an entry's first
+    // table word (its sample index) loads the entry's weight, in flight with
+    // the grid loads below
     constexpr bool kDcwE = MODE == 1;
-    float dj = 1.0f;
-    if (kDcwE && (uint32_t)tid < ne) dj = dcw_e[it.begin + tid];
+    const bool head = (tid & 3) == 0;
+    float dj = 1.0f, dj2 = 1.0f;
+    if (kDcwE && head && tlive) dj = dcw[tv.x];
+    if (kDcwE && head && tlive2) dj2 = dcw[tv2.x];
 
     // tile + halo: a thread owns one float (cell ex, re or im) of a staged row
     // and walks the E1 rows for 4 coils; the row wrap and centring sign are
@@ -357,7 +362,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     }
     if (tlive) st[tid] = tv;
     if (tlive2) st[tid + kTT] = tv2;
-    if (kDcwE && (uint32_t)tid < ne) sde[tid] = dj;
+    if (kDcwE && head && tlive) sde[tid >> 2] = dj;
+    if (kDcwE && head && tlive2) sde[(tid + kTT) >> 2] = dj2;
     __syncthreads();
 
     if constexpr (G == 1) {

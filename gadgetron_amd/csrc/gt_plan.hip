@@ -551,12 +551,7 @@ struct LaunchFastC2NC {
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
         const size_t lds = size_t(gt::c2nc_lds_bytes(KW, G));
         const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
-        if (mode == 1) {
-            // per-sample dcw in table order, staged with the table
-            b.dcw_e.reserve(std::max<size_t>(b.n, 1) * sizeof(float));
-            hipLaunchKernelGGL(gt::entry_dcw_kernel, dim3((unsigned)((b.n + 255) / 256)), dim3(256), 0, s,
-                               b.list.as<uint32_t>(), (long long)b.n, dcw, b.dcw_e.as<float>());
-        }
+        // a per-sample dcw is gathered per table entry inside the kernel
         auto k = G == 4 ? pick<4>(mode) : (G == 2 ? pick<2>(mode) : pick<1>(mode));
         hipLaunchKernelGGL(k, dim3((unsigned)b.n_items, chunks), dim3(64 * gt::kWaves), lds, s, grid, out,
                            b.table.as<uint32_t>(), b.items.as<gt::WorkItem>(), int(p->M), int(p->F), C, b.geom, L,
