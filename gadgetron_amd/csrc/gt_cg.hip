@@ -671,6 +671,7 @@ gt_status gt_sense_create(gt_sense_op** out, gt_nufft_plan* plan, const void* cs
         *out = nullptr;
         GT_REQUIRE(ncoils > 0, GT_ERR_INVALID_ARGUMENT, "no coils");
         GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
+        GT_REQUIRE(!plan_is_double(plan), GT_ERR_UNSUPPORTED, "the SENSE operator takes float (GT_FLOAT) plans");
         auto op = std::make_unique<gt_sense_op>();
         op->plan = plan;
         size_t N[4] = {0, 0, 0, 0};

@@ -94,5 +94,7 @@ struct DevBuf {
 // caller then multiplies the coil maps itself.
 bool forward_with_coil_maps(gt_nufft_plan* p, const void* x, const void* csm, size_t C, void* out,
                             const float* dcw, size_t dcw_len, hipStream_t s);
+// True for a GT_DOUBLE plan (the SENSE operator and CG solver take float plans).
+bool plan_is_double(const gt_nufft_plan* p);
 
 }  // namespace gt_host

@@ -74,6 +74,35 @@ __host__ __device__ inline float kb1_deriv(float u, float nos, float one_over_W,
 }
 
 // ConvolutionKernel.hpp:287-303, evaluated in float like the float reference.
+// Double forms (ConvolutionKernel.hpp:9-37 and :70-82): the same rational
+// approximant and kernel in double precision, for double-precision plans.
+__host__ __device__ inline double bessi0d(double x) {
+    if (x == 0.0) return 1.0;
+    const double z = x * x;
+    const double num =
+        (z * (z * (z * (z * (z * (z * (z * (z * (z * (z * (z * (z * (z *
+            (z * 0.210580722890567e-22 + 0.380715242345326e-19) +
+            0.479440257548300e-16) + 0.435125971262668e-13) +
+            0.300931127112960e-10) + 0.160224679395361e-7) +
+            0.654858370096785e-5) + 0.202591084143397e-2) +
+            0.463076284721000e0) + 0.754337328948189e2) +
+            0.830792541809429e4) + 0.571661130563785e6) +
+            0.216415572361227e8) + 0.356644482244025e9) +
+            0.144048298227235e10);
+    const double den = (z * (z * (z - 0.307646912682801e4) + 0.347626332405882e7) - 0.144048298227235e10);
+    return -num / den;
+}
+
+__host__ __device__ inline double kb1d(double u, double nos, double one_over_W, double beta) {
+    const double t = 2.0 * u * one_over_W;
+    const double arg = beta * sqrt(1.0 - t * t);
+    return nos * bessi0d(arg) * one_over_W;
+}
+
+inline double kb_beta_d(double W, double os_factor) {
+    return M_PI * std::sqrt(W * W / (os_factor * os_factor) * (os_factor - 0.5) * (os_factor - 0.5) - 0.8);
+}
+
 inline float kb_beta(float W, float os_factor) {
     auto sqr = [](float x) { return x * x; };
     return float(M_PI) * std::sqrt(sqr(W) / sqr(os_factor) * sqr(os_factor - 0.5f) - 0.8f);

@@ -41,6 +41,10 @@ struct GridGeom {
     float radius, one_over_W;
     float nos_f[3];
     float beta[3];
+    // the same in double, for double-precision plans
+    double radius_d, one_over_W_d;
+    double nos_d[3];
+    double beta_d[3];
 };
 
 struct WorkItem {

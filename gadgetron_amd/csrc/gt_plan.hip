@@ -41,6 +41,7 @@
 #include "gt_kernels_fast.h"
 #include "gt_kernels_reg.h"
 #include "gt_kernels_3d.h"
+#include "gt_kernels_d.h"
 #include "gt_fft.h"
 
 std::string& gt_host::last_error() {
@@ -149,16 +150,20 @@ struct gt_nufft_plan {
     bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
     bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
     bool fast3d = false;     // the 3D hot path (register-tile NC2C, staged C2NC)
+    bool dbl = false;        // GT_DOUBLE: the generic path in double precision
+    double radius_d = 0, one_over_W_d = 0, beta_d[3] = {0, 0, 0}, nos_d[3] = {1, 1, 1};
 
     gt::GridGeom geom{};  // geometry for the generic kernels
     std::vector<float> filt_host[3];
     DevBuf filt[3];
+    DevBuf filt_d[3];   // GT_DOUBLE: the filters in double
     DevBuf twiddle[2];  // exp(-2 pi i m / N_os[d]), m < N_os[d] (fused FFT)
 
     // preprocess state
     size_t M = 0, F = 0;
     bool prep_c2nc = false, prep_nc2c = false;
-    DevBuf traj;  // scaled positions [M*F][D]
+    DevBuf traj;  // scaled positions [M*F][D] (GT_DOUBLE: a float copy for the binning)
+    DevBuf traj_d;  // GT_DOUBLE: scaled positions in double
     DevBuf row_inv;     // SPARSE_MATRIX: 1 / window weight sum per sample [M*F]
     DevBuf sparse_dcw;  // SPARSE_MATRIX: caller's dcw times row_inv
     DevBuf flag;
@@ -204,6 +209,8 @@ struct gt_nufft_plan {
             kv.second.work.release();
         }
         for (auto& b : filt) b.release();
+        for (auto& b : filt_d) b.release();
+        traj_d.release();
         traj.release();
         flag.release();
         bn.release();
@@ -279,6 +286,26 @@ std::vector<float> deapod_1d(size_t nos, float radius, float one_over_W, float b
     return out;
 }
 
+// The same filter in double (GT_DOUBLE plans).
+std::vector<double> deapod_1d_d(size_t nos, double radius, double one_over_W, double beta) {
+    std::vector<double> h(nos);
+    for (size_t x = 0; x < nos; x++) {
+        const double r = std::fabs(double(int(x)) - double(nos) / 2.0);
+        h[x] = (r > radius) ? 0.0 : gt::kb1d(r, double(nos), one_over_W, beta);
+    }
+    std::vector<double> out(nos);
+    const double c = double(nos / 2);
+    for (size_t k = 0; k < nos; k++) {
+        double acc = 0.0;
+        for (size_t x = 0; x < nos; x++) {
+            if (h[x] == 0.0) continue;
+            acc += h[x] * std::cos(2.0 * M_PI * (double(k) - c) * (double(x) - c) / double(nos));
+        }
+        out[k] = 1.0 / acc;
+    }
+    return out;
+}
+
 void set_device(const gt_nufft_plan* p) { GT_HIP(hipSetDevice(p->device)); }
 
 void finish_fft_plan(FFTPlan& fp) {
@@ -306,9 +333,9 @@ FFTPlan& get_fft(gt_nufft_plan* p, int kind, size_t batch) {
     if (kind == FFT_IMAGE) {
         size_t lengths[3] = {p->Nos[0], p->Nos[1], p->Nos[2]};  // fastest dimension first
         GT_FFT(rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
-                                  rocfft_precision_single, p->D, lengths, batch, nullptr));
+                                  (p->dbl ? rocfft_precision_double : rocfft_precision_single), p->D, lengths, batch, nullptr));
         GT_FFT(rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
-                                  rocfft_precision_single, p->D, lengths, batch, nullptr));
+                                  (p->dbl ? rocfft_precision_double : rocfft_precision_single), p->D, lengths, batch, nullptr));
     } else if (kind == FFT_GRID2D) {
         // 2-D transforms of the [y][batch][x] layout: x stride 1, y stride batch*N_os[0],
         // one transform per batch at distance N_os[0]
@@ -322,10 +349,10 @@ FFTPlan& get_fft(gt_nufft_plan* p, int kind, size_t batch) {
             nullptr, 2, strides, dist, 2, strides, dist);
         if (st == rocfft_status_success)
             st = rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
-                                    rocfft_precision_single, 2, lengths, batch, desc);
+                                    (p->dbl ? rocfft_precision_double : rocfft_precision_single), 2, lengths, batch, desc);
         if (st == rocfft_status_success)
             st = rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
-                                    rocfft_precision_single, 2, lengths, batch, desc);
+                                    (p->dbl ? rocfft_precision_double : rocfft_precision_single), 2, lengths, batch, desc);
         rocfft_plan_description_destroy(desc);
         GT_FFT(st);
     } else {
@@ -348,10 +375,10 @@ FFTPlan& get_fft(gt_nufft_plan* p, int kind, size_t batch) {
             nullptr, 1, &stride, dist, 1, &stride, dist);
         if (st == rocfft_status_success)
             st = rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
-                                    rocfft_precision_single, 1, &len, count, desc);
+                                    (p->dbl ? rocfft_precision_double : rocfft_precision_single), 1, &len, count, desc);
         if (st == rocfft_status_success)
             st = rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
-                                    rocfft_precision_single, 1, &len, count, desc);
+                                    (p->dbl ? rocfft_precision_double : rocfft_precision_single), 1, &len, count, desc);
         rocfft_plan_description_destroy(desc);
         GT_FFT(st);
     }
@@ -492,6 +519,30 @@ struct LaunchNC2CAtomic {
     }
 };
 
+template <int D, int KW>
+struct LaunchC2NCD {
+    static void run(const double2* grid, double2* out, const double* pos, int M, int F, int C,
+                    const gt::GridGeom& g, int sign, const double* dcw, long long dcw_len, int accumulate,
+                    hipStream_t s) {
+        const long long n = (long long)M * F;
+        hipLaunchKernelGGL((gt::c2nc_gather_d_kernel<D, KW, 4>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           s, grid, out, pos, M, F, C, g, sign, dcw, dcw_len, accumulate);
+    }
+};
+template <int D, int KW>
+struct LaunchNC2CTileD {
+    static void run(const double2* samples, double2* grid, double2* partials, const double* pos,
+                    const uint32_t* tile_samples, const gt::WorkItem* items, int n_items, int M, int F, int C,
+                    int CC, const gt::GridGeom& g, int sign, const double* dcw, long long dcw_len, int accumulate,
+                    hipStream_t s) {
+        const unsigned chunks = (unsigned)((C + CC - 1) / CC);
+        const size_t lds = size_t(2) * CC * g.tile_cells * sizeof(double);
+        hipLaunchKernelGGL((gt::nc2c_tile_d_kernel<D, KW>), dim3((unsigned)n_items, chunks), dim3(64), lds, s,
+                           samples, grid, partials, pos, tile_samples, items, M, F, C, CC, g, sign, dcw, dcw_len,
+                           accumulate);
+    }
+};
+
 template <int KW>
 struct LaunchTablesNC2C {
     static void run(const float* pos, const uint32_t* list, const uint32_t* keys, long long n,
@@ -623,6 +674,8 @@ gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile, bool tile_covers
     g.D = p->D;
     g.radius = p->radius;
     g.one_over_W = p->one_over_W;
+    g.radius_d = p->radius_d;
+    g.one_over_W_d = p->one_over_W_d;
     g.G = (long long)p->G();
     g.ntiles_total = 1;
     g.tile_cells = 1;
@@ -631,6 +684,8 @@ gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile, bool tile_covers
             g.nos[d] = int(p->Nos[d]);
             g.nos_f[d] = float(unsigned(p->Nos[d]));
             g.beta[d] = p->beta[d];
+            g.nos_d[d] = double(p->Nos[d]);
+            g.beta_d[d] = p->beta_d[d];
             // the generic kernels: a window of KW cells touches at most two tiles
             // per dimension; the 3D hot path's tables take any tile
             const int t = std::min<int>(tile_covers_window ? std::max(tile[d], p->KW) : tile[d], int(p->Nos[d]));
@@ -639,6 +694,7 @@ gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile, bool tile_covers
         } else {
             g.nos[d] = 1;
             g.nos_f[d] = 1.0f;
+            g.nos_d[d] = 1.0;
             g.tile[d] = 1;
             g.ntiles[d] = 1;
         }
@@ -659,7 +715,7 @@ void setup_geometry(gt_nufft_plan* p) {
     // (the default; any other requested tile selects the generic kernels) and
     // a grid of at least tile + window cells per dimension.
     p->fast2d = false;
-    if (p->D == 2 && p->conv_type != GT_CONV_ATOMIC && p->KW >= 3 && p->KW <= gt::kTabKW) {
+    if (p->D == 2 && !p->dbl && p->conv_type != GT_CONV_ATOMIC && p->KW >= 3 && p->KW <= gt::kTabKW) {
         int ft[3];
         for (int d = 0; d < 3; d++) ft[d] = p->tile_req[d] > 0 ? p->tile_req[d] : gt::kRT;
         bool ok = true;
@@ -682,7 +738,7 @@ void setup_geometry(gt_nufft_plan* p) {
     // 3D hot path: STANDARD gridding, a window of 3..7 cells, no requested
     // tile, and a grid of at least tile + window cells per dimension.
     p->fast3d = false;
-    if (p->D == 3 && p->conv_type != GT_CONV_ATOMIC && p->KW >= 3 && p->KW <= gt::kTabKW &&
+    if (p->D == 3 && !p->dbl && p->conv_type != GT_CONV_ATOMIC && p->KW >= 3 && p->KW <= gt::kTabKW &&
         p->tile_req[0] <= 0 && p->tile_req[1] <= 0 && p->tile_req[2] <= 0) {
         const int tn[3] = {gt::k3T0, gt::k3T1, gt::k3T2}, tc[3] = {gt::k3S0, gt::k3S1, gt::k3S2};
         bool ok = true;
@@ -930,7 +986,11 @@ void build_nc2c(gt_nufft_plan* p, hipStream_t s) {
         keys_sorted.release();
     } else {
         b.geom = p->geom;
-        auto counts = bin_windows(p, b.geom, b, nullptr, s);
+        // double plans bin float copies of the positions with a widened window,
+        // so every tile the double-precision window touches is listed
+        gt::GridGeom gbin = p->geom;
+        if (p->dbl) gbin.radius += gt::kBinMargin;
+        auto counts = bin_windows(p, gbin, b, nullptr, s);
         make_items(p, b, counts, segment_length(p, counts, b.n, true), true, s);
     }
 }
@@ -1103,6 +1163,108 @@ void crop_deapod(gt_nufft_plan* p, const void* grid, void* img, size_t nb, bool 
                        dim3(64, gt::kRowsPerBlock), 0, s, (const float2*)grid, (float2*)img, (long long)nb, p->geom,
                        pc);
     GT_HIP(hipGetLastError());
+}
+
+// ---- double-precision plans: the generic path in double ---------------------
+gt::PadCropD padcrop_d(const gt_nufft_plan* p, bool pad_side, bool in_os, double scale, int sign) {
+    gt::PadCropD pc{};
+    for (int d = 0; d < 3; d++) {
+        pc.N[d] = d < p->D ? int(p->N[d]) : 1;
+        if (d < p->D) {
+            pc.off[d] = pad_side ? int(p->Nos[d] / 2 - p->N[d] / 2) : int((p->Nos[d] - p->N[d]) >> 1);
+            pc.filt[d] = p->filt_d[d].as<double>();
+        } else {
+            pc.off[d] = 0;
+            pc.filt[d] = nullptr;
+        }
+    }
+    pc.scale = scale;
+    pc.sign = sign;
+    pc.in_os = in_os ? 1 : 0;
+    return pc;
+}
+
+void pad_deapod_d(gt_nufft_plan* p, const void* img, void* grid, size_t nb, bool in_os, double scale,
+                  hipStream_t s) {
+    const gt::PadCropD pc = padcrop_d(p, true, in_os, scale, 1);
+    const long long rows = (long long)(nb * p->G() / p->Nos[0]);
+    StageTimer timer(p, GT_STAGE_PAD, s);
+    hipLaunchKernelGGL(gt::pad_deapod_d_kernel, dim3((unsigned)((rows + gt::kRowsPerBlock - 1) / gt::kRowsPerBlock)),
+                       dim3(64, gt::kRowsPerBlock), 0, s, (const double2*)img, (double2*)grid, (long long)nb, p->geom,
+                       pc);
+    GT_HIP(hipGetLastError());
+}
+
+void crop_deapod_d(gt_nufft_plan* p, const void* grid, void* img, size_t nb, bool out_os, double scale, int sign,
+                   hipStream_t s) {
+    const gt::PadCropD pc = padcrop_d(p, false, out_os, scale, sign);
+    const long long rows = (long long)(nb * (out_os ? p->G() / p->Nos[0] : p->Nimg() / p->N[0]));
+    StageTimer timer(p, GT_STAGE_CROP, s);
+    hipLaunchKernelGGL(gt::crop_deapod_d_kernel, dim3((unsigned)((rows + gt::kRowsPerBlock - 1) / gt::kRowsPerBlock)),
+                       dim3(64, gt::kRowsPerBlock), 0, s, (const double2*)grid, (double2*)img, (long long)nb, p->geom,
+                       pc);
+    GT_HIP(hipGetLastError());
+}
+
+void conv_c2nc_d(gt_nufft_plan* p, const void* grid, void* out, size_t C, int sign, const double* dcw,
+                 size_t dcw_len, bool accumulate, hipStream_t s) {
+    GT_REQUIRE(p->prep_c2nc, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for C2NC");
+    StageTimer timer(p, GT_STAGE_C2NC, s);
+    dispatch<LaunchC2NCD>(p->D, p->KW, (const double2*)grid, (double2*)out, p->traj_d.as<double>(), int(p->M),
+                          int(p->F), int(C), p->geom, sign, dcw, (long long)std::max<size_t>(dcw_len, 1),
+                          accumulate ? 1 : 0, s);
+    GT_HIP(hipGetLastError());
+}
+
+void conv_nc2c_d(gt_nufft_plan* p, const void* samples, void* grid, size_t C, int sign, const double* dcw,
+                 size_t dcw_len, bool accumulate, hipStream_t s) {
+    GT_REQUIRE(p->prep_nc2c, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for NC2C");
+    Binning& b = p->bn;
+    const int TC = std::max(1, b.geom.tile_cells);
+    if (b.n_partials > 0) p->partials.reserve(size_t(b.n_partials) * C * TC * sizeof(double2));
+    StageTimer timer(p, GT_STAGE_NC2C, s);
+    // LDS budget per one-wave workgroup: 16 KiB of re/im accumulators
+    const int CC = std::max(1, std::min(16384 / (16 * TC), 32));
+    const long long dl = (long long)std::max<size_t>(dcw_len, 1);
+    dispatch<LaunchNC2CTileD>(p->D, p->KW, (const double2*)samples, (double2*)grid, p->partials.as<double2>(),
+                              p->traj_d.as<double>(), b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), b.n_items,
+                              int(p->M), int(p->F), int(C), CC, b.geom, sign, dcw, dl, accumulate ? 1 : 0, s);
+    GT_HIP(hipGetLastError());
+    if (b.n_red > 0) {
+        hipLaunchKernelGGL(gt::nc2c_reduce_d_kernel, dim3((unsigned)b.n_red), dim3(256), 0, s,
+                           p->partials.as<double2>(), (double2*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                           b.geom, sign, accumulate ? 1 : 0);
+        GT_HIP(hipGetLastError());
+    }
+}
+
+// compute() of a double-precision plan (NFFT.hpp:104-238, modes as in compute_impl);
+// dcw points to doubles.
+void compute_impl_d(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const double* dcw, size_t dcw_len,
+                    int mode, hipStream_t s) {
+    const size_t G = p->G();
+    const double scale = double(centring_global_sign(p)) / std::sqrt(double(G));
+    switch (mode) {
+    case GT_FORWARDS_C2NC:
+    case GT_BACKWARDS_C2NC:
+        p->grid.reserve(sh.nb * G * sizeof(double2));
+        pad_deapod_d(p, in, p->grid.ptr, sh.nb, sh.oversampled, scale, s);
+        run_fft(p, p->grid.ptr, sh.nb, mode == GT_FORWARDS_C2NC, s);
+        conv_c2nc_d(p, p->grid.ptr, out, sh.C, 1, dcw, dcw_len, false, s);
+        break;
+    case GT_BACKWARDS_NC2C:
+    case GT_FORWARDS_NC2C: {
+        void* g = out;
+        if (!sh.oversampled) {
+            p->grid.reserve(sh.nb * G * sizeof(double2));
+            g = p->grid.ptr;
+        }
+        conv_nc2c_d(p, in, g, sh.C, 1, dcw, dcw_len, false, s);
+        run_fft(p, g, sh.nb, mode == GT_FORWARDS_NC2C, s);
+        crop_deapod_d(p, g, out, sh.nb, sh.oversampled, scale, 1, s);
+        break;
+    }
+    }
 }
 
 gt::RowGeom row_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float scale) {
@@ -1280,6 +1442,10 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
     const size_t selems = image_in ? elements(out_dims, out_ndim) : elements(in_dims, in_ndim);
     check_batches(p, sh, selems);
     set_device(p);
+    if (p->dbl) {
+        compute_impl_d(p, in, out, sh, reinterpret_cast<const double*>(dcw), dcw_len, mode, s);
+        return;
+    }
     const size_t G = p->G();
     const float gs = centring_global_sign(p);
     const float scale = gs / std::sqrt(float(G));  // do_scale=true (NFFT.h:114-115, NFFT.hpp:205-238)
@@ -1318,6 +1484,8 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
 
 }  // namespace
 
+bool gt_host::plan_is_double(const gt_nufft_plan* p) { return p->dbl; }
+
 bool gt_host::forward_with_coil_maps(gt_nufft_plan* p, const void* x, const void* csm, size_t C, void* out,
                                      const float* dcw, size_t dcw_len, hipStream_t s) {
     if (!(p->fast2d && p->fft_fused && fft2d_strategy() == 2)) return false;
@@ -1346,8 +1514,11 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         GT_REQUIRE(out && d, GT_ERR_INVALID_ARGUMENT, "null plan or descriptor");
         *out = nullptr;
         GT_REQUIRE(d->D >= 1 && d->D <= 3, GT_ERR_UNSUPPORTED, "D must be 1, 2 or 3");
-        GT_REQUIRE(d->precision == GT_FLOAT, GT_ERR_UNSUPPORTED,
-                   "only single precision (complext<float>) is provided");
+        GT_REQUIRE(d->precision == GT_FLOAT || d->precision == GT_DOUBLE, GT_ERR_UNSUPPORTED,
+                   "precision must be GT_FLOAT or GT_DOUBLE");
+        // NFFT<cuNDArray, double, D>::make_plan (cuNFFT.cu:219-227)
+        GT_REQUIRE(d->precision == GT_FLOAT || d->conv_type == GT_CONV_STANDARD, GT_ERR_UNSUPPORTED,
+                   "Only standard convolution type supported for doubles");
         GT_REQUIRE(d->conv_type == GT_CONV_STANDARD || d->conv_type == GT_CONV_ATOMIC ||
                        d->conv_type == GT_CONV_SPARSE_MATRIX, GT_ERR_INVALID_ARGUMENT,
                    "Invalid convolution type provided.");
@@ -1362,18 +1533,24 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         p->radius = p->Wf / 2.0f;
         p->one_over_W = 1.0f / p->Wf;
         p->conv_type = d->conv_type;
+        p->dbl = d->precision == GT_DOUBLE;
+        p->radius_d = d->W / 2.0;
+        p->one_over_W_d = 1.0 / d->W;
         for (int i = 0; i < p->D; i++) {
             GT_REQUIRE(d->N[i] >= 1, GT_ERR_INVALID_ARGUMENT, "matrix size must be positive");
             p->N[i] = d->N[i];
             float osf;
+            double osf_d;
             if (d->os_factor != 0.0) {
-                // NFFT.hpp:23-37: N_os = size_t(alpha * N); KaiserKernel os ctor (ConvolutionKernel.hpp:241-254)
+                // NFFT.hpp:23-37: N_os = size_t(alpha * N) in REAL; KaiserKernel os ctor (ConvolutionKernel.hpp:241-254)
                 const float a = float(d->os_factor);
-                p->Nos[i] = size_t(a * float(d->N[i]));
+                p->Nos[i] = p->dbl ? size_t(d->os_factor * double(d->N[i])) : size_t(a * float(d->N[i]));
                 osf = a;
+                osf_d = d->os_factor;
             } else {
                 p->Nos[i] = d->N_os[i];
                 osf = float(unsigned(d->N_os[i])) / float(unsigned(d->N[i]));
+                osf_d = double(unsigned(d->N_os[i])) / double(unsigned(d->N[i]));
             }
             GT_REQUIRE(p->Nos[i] >= p->N[i], GT_ERR_INVALID_ARGUMENT,
                        "Oversampled matrix size must be greater than or equal to matrix size.");
@@ -1382,6 +1559,8 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
             GT_REQUIRE(double(p->Nos[i]) > d->W, GT_ERR_INVALID_ARGUMENT,
                        "kernel width must be smaller than the oversampled matrix size");
             p->beta[i] = gt::kb_beta(p->Wf, osf);
+            p->beta_d[i] = gt::kb_beta_d(d->W, osf_d);
+            p->nos_d[i] = double(p->Nos[i]);
             GT_REQUIRE(std::isfinite(p->beta[i]), GT_ERR_INVALID_ARGUMENT,
                        "Kaiser-Bessel beta is not finite for this width and oversampling factor");
         }
@@ -1397,10 +1576,15 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
             p->filt[i].reserve(p->Nos[i] * sizeof(float));
             GT_HIP(hipMemcpy(p->filt[i].ptr, p->filt_host[i].data(), p->Nos[i] * sizeof(float),
                              hipMemcpyHostToDevice));
+            if (p->dbl) {
+                const std::vector<double> fd = deapod_1d_d(p->Nos[i], p->radius_d, p->one_over_W_d, p->beta_d[i]);
+                p->filt_d[i].reserve(p->Nos[i] * sizeof(double));
+                GT_HIP(hipMemcpy(p->filt_d[i].ptr, fd.data(), p->Nos[i] * sizeof(double), hipMemcpyHostToDevice));
+            }
         }
         p->flag.reserve(sizeof(int));
         setup_geometry(p.get());
-        if (p->D == 2 && fused_fft_len(int(p->Nos[0])) && fused_fft_len(int(p->Nos[1]))) {
+        if (p->D == 2 && !p->dbl && fused_fft_len(int(p->Nos[0])) && fused_fft_len(int(p->Nos[1]))) {
             p->fft_fused = true;
             for (int i = 0; i < 2; i++) {
                 const size_t L = p->Nos[i];
@@ -1459,9 +1643,17 @@ gt_status gt_nufft_preprocess(gt_nufft_plan* p, const void* traj, size_t M, size
         const long long n = (long long)(M * F * p->D);
         p->traj.reserve(size_t(n) * sizeof(float));
         GT_HIP(hipMemsetAsync(p->flag.ptr, 0, sizeof(int), s));
-        hipLaunchKernelGGL(gt::scale_traj_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           (const float*)traj, p->traj.as<float>(), n, p->D, p->geom.nos_f[0],
-                           p->geom.nos_f[1], p->geom.nos_f[2], p->flag.as<int>());
+        if (p->dbl) {
+            // traj: double D-vectors; positions in double, a float copy for the binning
+            p->traj_d.reserve(size_t(n) * sizeof(double));
+            hipLaunchKernelGGL(gt::scale_traj_d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               (const double*)traj, p->traj_d.as<double>(), p->traj.as<float>(), n, p->D,
+                               p->geom.nos_d[0], p->geom.nos_d[1], p->geom.nos_d[2], p->flag.as<int>());
+        } else {
+            hipLaunchKernelGGL(gt::scale_traj_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               (const float*)traj, p->traj.as<float>(), n, p->D, p->geom.nos_f[0],
+                               p->geom.nos_f[1], p->geom.nos_f[2], p->flag.as<int>());
+        }
         GT_HIP(hipGetLastError());
         int bad = 0;
         GT_HIP(hipMemcpyAsync(&bad, p->flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1503,7 +1695,7 @@ gt_status gt_nufft_mult_MH_M(gt_nufft_plan* p, const void* in, void* out, const 
         GT_REQUIRE(p->M > 0, GT_ERR_NOT_PREPROCESSED, "plan has not been preprocessed");
         Shape sh = image_shape(p, dims, ndim, true);
         const size_t nb = sh.nb;
-        p->tmp_samples.reserve(std::max<size_t>(nb * p->M, 1) * sizeof(float2));
+        p->tmp_samples.reserve(std::max<size_t>(nb * p->M, 1) * (p->dbl ? sizeof(double2) : sizeof(float2)));
         const size_t sd[1] = {nb * p->M};
         compute_impl(p, in, dims, ndim, p->tmp_samples.ptr, sd, 1, dcw, dcw_len, GT_FORWARDS_C2NC, S(stream));
         compute_impl(p, p->tmp_samples.ptr, sd, 1, out, dims, ndim, dcw, dcw_len, GT_BACKWARDS_NC2C, S(stream));
@@ -1522,8 +1714,14 @@ gt_status gt_nufft_convolve(gt_nufft_plan* p, const void* in, const size_t* in_d
         check_batches(p, sh, c2nc ? elements(out_dims, out_ndim) : elements(in_dims, in_ndim));
         set_device(p);
         const gt::Layout L = caller_layout(p);
-        if (c2nc) conv_c2nc(p, in, L, out, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
-        else conv_nc2c(p, in, out, L, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
+        if (p->dbl) {
+            if (c2nc) conv_c2nc_d(p, in, out, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
+            else conv_nc2c_d(p, in, out, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
+        } else if (c2nc) {
+            conv_c2nc(p, in, L, out, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
+        } else {
+            conv_nc2c(p, in, out, L, sh.C, 0, nullptr, 0, accumulate != 0, S(stream));
+        }
     });
 }
 
@@ -1537,6 +1735,18 @@ gt_status gt_nufft_fft(gt_nufft_plan* p, void* data, const size_t* dims, int ndi
         set_device(p);
         hipStream_t s = S(stream);
         const long long total = (long long)(sh.nb * p->G());
+        if (p->dbl) {
+            const double gsd = centring_global_sign(p);
+            const double sd = do_scale ? gsd / std::sqrt(double(p->G())) : gsd;
+            hipLaunchKernelGGL(gt::sign_scale_d_kernel, dim3(grid_blocks(total)), dim3(256), 0, s, (double2*)data,
+                               (long long)sh.nb, p->geom, 1.0, 1);
+            GT_HIP(hipGetLastError());
+            run_fft(p, data, sh.nb, fft_mode == GT_FFT_FORWARDS, s);
+            hipLaunchKernelGGL(gt::sign_scale_d_kernel, dim3(grid_blocks(total)), dim3(256), 0, s, (double2*)data,
+                               (long long)sh.nb, p->geom, sd, 1);
+            GT_HIP(hipGetLastError());
+            return;
+        }
         const float gs = centring_global_sign(p);
         const float scale = do_scale ? gs / std::sqrt(float(p->G())) : gs;
         // centred FFT = (-1)^(sum N_os/2) * T . FFT . T  (cuNDFFT.cpp:124-138 timeswitch pair)
@@ -1562,6 +1772,10 @@ gt_status gt_nufft_deapodize(gt_nufft_plan* p, void* image, const size_t* dims, 
             throw gt_error(GT_ERR_SHAPE, "Error: cuNFFT_impl::deapodize: ERROR: oversampled image not provided as input.");
         }
         set_device(p);
+        if (p->dbl) {
+            crop_deapod_d(p, image, image, sh.nb, true, 1.0, 0, S(stream));  // in place, as below
+            return;
+        }
         gt::PadCrop pc = padcrop(p, false, true, 1.0f, 0);
         const long long rows = (long long)(sh.nb * p->G() / p->Nos[0]);  // in place: each thread rewrites what it read
         hipLaunchKernelGGL(gt::crop_deapod_kernel, dim3((unsigned)((rows + gt::kRowsPerBlock - 1) / gt::kRowsPerBlock)),
@@ -1576,6 +1790,7 @@ gt_status gt_nufft_grad_traj(gt_nufft_plan* p, const void* image, const size_t* 
                              size_t dcw_len, float* grad_traj, gt_stream_t stream) {
     return guarded([&] {
         GT_REQUIRE(p && image && grad_samples && grad_traj, GT_ERR_INVALID_ARGUMENT, "null argument");
+        GT_REQUIRE(!p->dbl, GT_ERR_UNSUPPORTED, "the trajectory gradient is provided for float plans");
         GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
         GT_REQUIRE(p->M > 0 && p->F > 0, GT_ERR_NOT_PREPROCESSED, "plan has not been preprocessed");
         Shape sh = image_shape(p, dims, ndim, true);

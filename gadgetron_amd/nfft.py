@@ -137,11 +137,16 @@ class NFFTPlan:
     """cuNFFT_plan<float, D> (toolboxes/nfft/gpu/cuNFFT.h:52-131)."""
 
     def __init__(self, matrix_size, matrix_size_os=None, W=5.5, os_factor=None,
-                 conv=ConvolutionType.STANDARD, device=-1):
+                 conv=ConvolutionType.STANDARD, device=-1, precision="float"):
+        """precision "double" makes an NFFT<cuNDArray, double, D> plan (STANDARD
+        only): float64 trajectories and dcw, complex128 samples and images."""
         D = len(matrix_size)
         desc = gt_nufft_desc()
         desc.D = D
-        desc.precision = 0
+        if precision not in ("float", "double"):
+            raise ValueError("precision must be 'float' or 'double'")
+        desc.precision = 0 if precision == "float" else 1
+        self.precision = precision
         for i, n in enumerate(matrix_size):
             desc.N[i] = int(n)
         if matrix_size_os is not None:

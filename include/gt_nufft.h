@@ -70,7 +70,11 @@ typedef struct gt_nufft_plan gt_nufft_plan;
  * cuNFFT.cu:170-202), which equals hoNFFT_plan(N, alpha, W) (hoNFFT.cpp:175-197). */
 typedef struct gt_nufft_desc {
     int D;               /* 1, 2 or 3 */
-    int precision;       /* GT_FLOAT (GT_DOUBLE is rejected: GT_ERR_UNSUPPORTED) */
+    int precision;       /* GT_FLOAT, or GT_DOUBLE (NFFT<cuNDArray, double, D>, cuNFFT.cu:219-227):
+                            STANDARD convolution only, the generic kernels in double;
+                            trajectories are double D-vectors, samples and images
+                            complex double, and a dcw argument points to doubles
+                            (passed through the `const float*` parameter) */
     size_t N[4];
     size_t N_os[4];
     double os_factor;

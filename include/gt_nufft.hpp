@@ -88,7 +88,8 @@ std::vector<size_t> dims_of(const A& a) {
 }
 }  // namespace detail
 
-// cuNFFT_plan<REAL, D>: REAL = float (double is rejected by the library).
+// cuNFFT_plan<REAL, D>: REAL = float, or double (STANDARD convolution only; the
+// DCW array of a double plan holds doubles).
 template <class REAL, unsigned int D>
 class cuNFFT_plan {
     static_assert(D >= 1 && D <= 3, "D must be 1, 2 or 3");

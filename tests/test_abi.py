@@ -52,7 +52,8 @@ def _desc(**kw):
     (dict(W=float("nan")), 1),
     (dict(D=4), 5),
     (dict(D=0), 5),
-    (dict(precision=1), 5),                 # double not provided
+    (dict(precision=1, conv_type=1), 5),    # double plans are STANDARD only
+    (dict(precision=2), 5),                 # no such precision
     (dict(conv_type=7), 1),                 # not a ConvolutionType
     (dict(N_os=[100, 192]), 1),             # N_os < N
     (dict(N_os=[193, 192]), 5),             # odd N_os
