@@ -681,14 +681,15 @@ gt_status gt_sense_create(gt_sense_op** out, gt_nufft_plan* plan, const void* cs
         chk(gt_nufft_get_matrix_size(plan, N));
         chk(gt_nufft_get_num_samples_frames(plan, &op->M, &op->F));
         GT_REQUIRE(op->M > 0 && op->F > 0, GT_ERR_NOT_PREPROCESSED, "plan has not been preprocessed");
-        op->D = 0;
+        int D = 0;
+        while (D < 4 && N[D] > 0) D++;
+        GT_REQUIRE(D >= 2 && D <= 3, GT_ERR_UNSUPPORTED, "the SENSE operator needs a 2D or 3D plan");
+        op->D = D;
         op->nimg = 1;
-        for (int d = 0; d < 3 && N[d] > 0; d++) {
+        for (int d = 0; d < D; d++) {
             op->N[d] = N[d];
             op->nimg *= N[d];
-            op->D = d + 1;
         }
-        GT_REQUIRE(op->D >= 2, GT_ERR_UNSUPPORTED, "the SENSE operator needs a 2D or 3D plan");
         op->C = ncoils;
         op->csm = static_cast<const float2*>(csm);
         op->dcw = dcw;

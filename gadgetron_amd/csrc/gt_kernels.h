@@ -32,19 +32,19 @@ namespace gt {
 
 struct GridGeom {
     int D;
-    int nos[3];        // oversampled grid size per dim (1 for unused dims)
-    int tile[3];       // tile edge per dim (1 for unused dims)
-    int ntiles[3];     // tiles per dim
+    int nos[4];        // oversampled grid size per dim (1 for unused dims)
+    int tile[4];       // tile edge per dim (1 for unused dims)
+    int ntiles[4];     // tiles per dim
     int ntiles_total;  // product of ntiles
     int tile_cells;    // product of tile
     long long G;       // cells per grid
     float radius, one_over_W;
-    float nos_f[3];
-    float beta[3];
+    float nos_f[4];
+    float beta[4];
     // the same in double, for double-precision plans
     double radius_d, one_over_W_d;
-    double nos_d[3];
-    double beta_d[3];
+    double nos_d[4];
+    double beta_d[4];
 };
 
 struct WorkItem {
@@ -98,15 +98,13 @@ __device__ __forceinline__ void window1d(float p, const GridGeom& g, int d, bool
 // (cuGriddingConvolution.cu:137-152).
 // ---------------------------------------------------------------------------
 __global__ void scale_traj_kernel(const float* __restrict__ traj, float* __restrict__ out,
-                                  long long n, int D, float nos0, float nos1, float nos2,
-                                  int* __restrict__ flag) {
+                                  long long n, int D, GridGeom g, int* __restrict__ flag) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int d = (int)(i % D);
     const float k = traj[i];
-    const float nos = d == 0 ? nos0 : (d == 1 ? nos1 : nos2);
     if (!(k >= -0.5f && k <= 0.5f)) flag[0] = 1;
-    out[i] = (k + 0.5f) * nos;
+    out[i] = (k + 0.5f) * g.nos_f[d];
 }
 
 // SPARSE_MATRIX row normalisation (ConvolverNC2C_sparse.cuh:84-111): one over
@@ -162,7 +160,7 @@ __global__ void count_tiles_kernel(const float* __restrict__ p, long long nsamp,
                                    uint32_t* __restrict__ counts) {
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsamp) return;
-    int t[3][4];
+    int t[4][4];
     uint32_t cnt = 1;
     for (int d = 0; d < g.D; d++) cnt *= (uint32_t)tiles1d(p[j * g.D + d], g, d, t[d]);
     counts[j] = cnt;
@@ -175,21 +173,25 @@ __global__ void emit_pairs_kernel(const float* __restrict__ p, long long nsamp, 
                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsamp) return;
-    int t[3][4];
-    int n[3] = {1, 1, 1};
+    int t[4][4];
+    int n[4] = {1, 1, 1, 1};
     for (int d = 0; d < g.D; d++) n[d] = tiles1d(p[j * g.D + d], g, d, t[d]);
-    for (int d = g.D; d < 3; d++) t[d][0] = 0;
+    for (int d = g.D; d < 4; d++) t[d][0] = 0;
     const uint32_t frame = (uint32_t)(j / M);
     uint32_t o = offsets[j];
-    for (int a = 0; a < n[2]; a++)
-        for (int b = 0; b < n[1]; b++)
-            for (int c = 0; c < n[0]; c++) {
-                const uint32_t tile = (uint32_t)t[0][c] +
-                                      (uint32_t)g.ntiles[0] * ((uint32_t)t[1][b] + (uint32_t)g.ntiles[1] * (uint32_t)t[2][a]);
-                keys[o] = frame * (uint32_t)g.ntiles_total + tile;
-                vals[o] = (uint32_t)j;
-                o++;
-            }
+    for (int v = 0; v < n[3]; v++)
+        for (int a = 0; a < n[2]; a++)
+            for (int b = 0; b < n[1]; b++)
+                for (int c = 0; c < n[0]; c++) {
+                    const uint32_t tile =
+                        (uint32_t)t[0][c] +
+                        (uint32_t)g.ntiles[0] * ((uint32_t)t[1][b] +
+                                                 (uint32_t)g.ntiles[1] * ((uint32_t)t[2][a] +
+                                                                          (uint32_t)g.ntiles[2] * (uint32_t)t[3][v]));
+                    keys[o] = frame * (uint32_t)g.ntiles_total + tile;
+                    vals[o] = (uint32_t)j;
+                    o++;
+                }
 }
 
 // Entry ranges of a sorted key array: first[k] / last[k] bound the run of key k
@@ -225,11 +227,12 @@ __global__ __launch_bounds__(256) void c2nc_gather_kernel(
     if (j >= nsamp) return;
     const int f = (int)(j / M);
 
-    float wx[KW], wy[KW], wz[KW];
-    int cx[KW], cy[KW], cz[KW];
+    float wx[KW], wy[KW], wz[KW], wv[KW];
+    int cx[KW], cy[KW], cz[KW], cv[KW];
     window1d<KW>(p[j * D + 0], g, 0, sign != 0, wx, cx);
     if (D >= 2) window1d<KW>(p[j * D + 1], g, 1, sign != 0, wy, cy);
     if (D >= 3) window1d<KW>(p[j * D + 2], g, 2, sign != 0, wz, cz);
+    if (D >= 4) window1d<KW>(p[j * D + 3], g, 3, sign != 0, wv, cv);
 
     const long long G = g.G;
     const long long FG = (long long)F * G;
@@ -240,8 +243,10 @@ __global__ __launch_bounds__(256) void c2nc_gather_kernel(
         float ar[CC], ai[CC];
 #pragma unroll
         for (int q = 0; q < CC; q++) { ar[q] = 0.0f; ai[q] = 0.0f; }
+        const int nv = D >= 4 ? KW : 1;
         const int nz = D >= 3 ? KW : 1;
         const int ny = D >= 2 ? KW : 1;
+        for (int iv = 0; iv < nv; iv++)
         for (int iz = 0; iz < nz; iz++) {
             for (int iy = 0; iy < ny; iy++) {
 #pragma unroll
@@ -254,9 +259,13 @@ __global__ __launch_bounds__(256) void c2nc_gather_kernel(
                     } else if (D == 2) {
                         w = wx[ix] * wy[iy];
                         cell = cx[ix] + (long long)g.nos[0] * cy[iy];
-                    } else {
+                    } else if (D == 3) {
                         w = (wx[ix] * wy[iy]) * wz[iz];
                         cell = cx[ix] + (long long)g.nos[0] * (cy[iy] + (long long)g.nos[1] * cz[iz]);
+                    } else {
+                        w = ((wx[ix] * wy[iy]) * wz[iz]) * wv[iv];
+                        cell = cx[ix] + (long long)g.nos[0] *
+                                            (cy[iy] + (long long)g.nos[1] * (cz[iz] + (long long)g.nos[2] * cv[iv]));
                     }
                     if (w != 0.0f) {
 #pragma unroll
@@ -411,7 +420,7 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
 
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     int t = (int)(it.key % (uint32_t)g.ntiles_total);
-    int org[3] = {0, 0, 0};
+    int org[4] = {0, 0, 0, 0};
     for (int d = 0; d < D; d++) {
         org[d] = (t % g.ntiles[d]) * g.tile[d];
         t /= g.ntiles[d];
@@ -422,11 +431,12 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
         const uint32_t e = e0 + threadIdx.x;
         const bool active = e < it.end;
         const uint32_t j = active ? tile_samples[e] : 0u;
-        float wx[KW], wy[KW], wz[KW];
-        int cx[KW], cy[KW], cz[KW];
+        float wx[KW], wy[KW], wz[KW], wv[KW];
+        int cx[KW], cy[KW], cz[KW], cv[KW];
         window1d<KW>(p[(long long)j * D + 0], g, 0, false, wx, cx);
         if (D >= 2) window1d<KW>(p[(long long)j * D + 1], g, 1, false, wy, cy);
         if (D >= 3) window1d<KW>(p[(long long)j * D + 2], g, 2, false, wz, cz);
+        if (D >= 4) window1d<KW>(p[(long long)j * D + 3], g, 3, false, wv, cv);
         // local in-tile coordinates; -1 marks cells outside this tile
 #pragma unroll
         for (int i = 0; i < KW; i++) {
@@ -440,6 +450,10 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
                 l = cz[i] - org[2];
                 cz[i] = (wz[i] != 0.0f && l >= 0 && l < g.tile[2]) ? l : -1;
             }
+            if (D >= 4) {
+                l = cv[i] - org[3];
+                cv[i] = (wv[i] != 0.0f && l >= 0 && l < g.tile[3]) ? l : -1;
+            }
         }
         for (int q = 0; q < ncc; q++) {
             const long long sidx = (long long)(c0 + q) * FM + j;
@@ -451,9 +465,12 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
             }
             float* are = acc_re + q * TC;
             float* aim = acc_im + q * TC;
+            const int nv = D >= 4 ? KW : 1;
             const int nz = D >= 3 ? KW : 1;
             const int ny = D >= 2 ? KW : 1;
+            for (int iv = 0; iv < nv; iv++)
             for (int iz = 0; iz < nz; iz++) {
+                if (D >= 4 && cv[iv] < 0) break;  // the same for every iz
                 if (D >= 3 && cz[iz] < 0) continue;
                 for (int iy = 0; iy < ny; iy++) {
                     if (D >= 2 && cy[iy] < 0) continue;
@@ -468,9 +485,12 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
                         } else if (D == 2) {
                             w = wx[ix] * wy[iy];
                             lc = cx[ix] + g.tile[0] * cy[iy];
-                        } else {
+                        } else if (D == 3) {
                             w = (wx[ix] * wy[iy]) * wz[iz];
                             lc = cx[ix] + g.tile[0] * (cy[iy] + g.tile[1] * cz[iz]);
+                        } else {
+                            w = ((wx[ix] * wy[iy]) * wz[iz]) * wv[iv];
+                            lc = cx[ix] + g.tile[0] * (cy[iy] + g.tile[1] * (cz[iz] + g.tile[2] * cv[iv]));
                         }
                         atomicAdd(&are[lc], s.x * w);
                         atomicAdd(&aim[lc], s.y * w);
@@ -490,7 +510,7 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
             partials[((long long)it.partial * C + (c0 + q)) * TC + lc] = v;
             continue;
         }
-        int gc[3] = {0, 0, 0};
+        int gc[4] = {0, 0, 0, 0};
         bool inside = true;
         int par = 0;
         for (int d = 0; d < D; d++) {
@@ -501,7 +521,8 @@ __global__ __launch_bounds__(64) void nc2c_tile_kernel(
         }
         if (!inside) continue;
         if (sign && (par & 1)) { v.x = -v.x; v.y = -v.y; }
-        const long long cell = gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * gc[2]);
+        const long long cell =
+            gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * (gc[2] + (long long)g.nos[2] * gc[3]));
         const long long o = ((long long)(c0 + q) * F + f) * G + cell;
         if (accumulate) {
             const float2 old = grid[o];
@@ -519,7 +540,7 @@ __global__ __launch_bounds__(256) void nc2c_reduce_kernel(
     const int TC = g.tile_cells;
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     int t = (int)(it.key % (uint32_t)g.ntiles_total);
-    int org[3] = {0, 0, 0};
+    int org[4] = {0, 0, 0, 0};
     for (int d = 0; d < g.D; d++) {
         org[d] = (t % g.ntiles[d]) * g.tile[d];
         t /= g.ntiles[d];
@@ -535,7 +556,7 @@ __global__ __launch_bounds__(256) void nc2c_reduce_kernel(
             v.y += a.y;
         }
         int lc = lc0;
-        int gc[3] = {0, 0, 0};
+        int gc[4] = {0, 0, 0, 0};
         bool inside = true;
         int par = 0;
         for (int d = 0; d < g.D; d++) {
@@ -546,7 +567,8 @@ __global__ __launch_bounds__(256) void nc2c_reduce_kernel(
         }
         if (!inside) continue;
         if (sign && (par & 1)) { v.x = -v.x; v.y = -v.y; }
-        const long long cell = gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * gc[2]);
+        const long long cell =
+            gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * (gc[2] + (long long)g.nos[2] * gc[3]));
         const long long o = ((long long)q * F + f) * G + cell;
         if (accumulate) {
             const float2 old = grid[o];
@@ -569,11 +591,12 @@ __global__ __launch_bounds__(256) void nc2c_atomic_kernel(
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsamp) return;
     const int f = (int)(j / M);
-    float wx[KW], wy[KW], wz[KW];
-    int cx[KW], cy[KW], cz[KW];
+    float wx[KW], wy[KW], wz[KW], wv[KW];
+    int cx[KW], cy[KW], cz[KW], cv[KW];
     window1d<KW>(p[j * D + 0], g, 0, sign != 0, wx, cx);
     if (D >= 2) window1d<KW>(p[j * D + 1], g, 1, sign != 0, wy, cy);
     if (D >= 3) window1d<KW>(p[j * D + 2], g, 2, sign != 0, wz, cz);
+    if (D >= 4) window1d<KW>(p[j * D + 3], g, 3, sign != 0, wv, cv);
     const long long G = g.G;
     for (int q = 0; q < C; q++) {
         const long long sidx = (long long)q * nsamp + j;
@@ -584,8 +607,10 @@ __global__ __launch_bounds__(256) void nc2c_atomic_kernel(
             s.y *= dw;
         }
         float* gq = (float*)(grid + ((long long)q * F + f) * G);
+        const int nv = D >= 4 ? KW : 1;
         const int nz = D >= 3 ? KW : 1;
         const int ny = D >= 2 ? KW : 1;
+        for (int iv = 0; iv < nv; iv++)
         for (int iz = 0; iz < nz; iz++)
             for (int iy = 0; iy < ny; iy++)
 #pragma unroll
@@ -598,9 +623,13 @@ __global__ __launch_bounds__(256) void nc2c_atomic_kernel(
                     } else if (D == 2) {
                         w = wx[ix] * wy[iy];
                         cell = cx[ix] + (long long)g.nos[0] * cy[iy];
-                    } else {
+                    } else if (D == 3) {
                         w = (wx[ix] * wy[iy]) * wz[iz];
                         cell = cx[ix] + (long long)g.nos[0] * (cy[iy] + (long long)g.nos[1] * cz[iz]);
+                    } else {
+                        w = ((wx[ix] * wy[iy]) * wz[iz]) * wv[iv];
+                        cell = cx[ix] + (long long)g.nos[0] *
+                                            (cy[iy] + (long long)g.nos[1] * (cz[iz] + (long long)g.nos[2] * cv[iv]));
                     }
                     if (w != 0.0f) {
                         unsafeAtomicAdd(gq + 2 * cell, s.x * w);
@@ -616,9 +645,9 @@ __global__ __launch_bounds__(256) void nc2c_atomic_kernel(
 // filt[d] holds the reciprocal 1-D deapodization filter of dim d (N_os[d]).
 // ---------------------------------------------------------------------------
 struct PadCrop {
-    int N[3];
-    int off[3];
-    const float* filt[3];
+    int N[4];
+    int off[4];
+    const float* filt[4];
     float scale;
     int sign;
     int in_os;
@@ -633,25 +662,30 @@ __global__ __launch_bounds__(64 * kRowsPerBlock) void pad_deapod_kernel(const fl
                                                                         float2* __restrict__ grid, long long nb,
                                                                         GridGeom g, PadCrop pc) {
     const long long R = (long long)blockIdx.x * kRowsPerBlock + threadIdx.y;
-    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2];
-    if (R >= nb * Oy * Oz) return;
+    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2], Ov = g.nos[3];
+    if (R >= nb * Oy * Oz * Ov) return;
     const int y = (int)(R % Oy);
     const long long t = R / Oy;
     const int z = (int)(t % Oz);
-    const long long b = t / Oz;
+    const long long t2 = t / Oz;
+    const int u = (int)(t2 % Ov);
+    const long long b = t2 / Ov;
     float2* out = grid + R * Ox;
     const int iy = pc.in_os ? y : y - pc.off[1], iz = pc.in_os ? z : z - pc.off[2];
+    const int iv = pc.in_os ? u : u - pc.off[3];
     const int Nx = pc.in_os ? Ox : pc.N[0], Ny = pc.in_os ? Oy : pc.N[1], Nz = pc.in_os ? Oz : pc.N[2];
+    const int Nv = pc.in_os ? Ov : pc.N[3];
     const int offx = pc.in_os ? 0 : pc.off[0];
-    if (iy < 0 || iy >= Ny || iz < 0 || iz >= Nz) {
+    if (iy < 0 || iy >= Ny || iz < 0 || iz >= Nz || iv < 0 || iv >= Nv) {
         for (int x = threadIdx.x; x < Ox; x += 64) out[x] = make_float2(0.0f, 0.0f);
         return;
     }
-    const float2* in = img + ((b * Nz + iz) * Ny + iy) * Nx;
+    const float2* in = img + (((b * Nv + iv) * Nz + iz) * Ny + iy) * Nx;
     float sr = pc.scale;
     if (g.D >= 2) sr *= pc.filt[1][y];
     if (g.D >= 3) sr *= pc.filt[2][z];
-    if (pc.sign && ((y + z) & 1)) sr = -sr;
+    if (g.D >= 4) sr *= pc.filt[3][u];
+    if (pc.sign && ((y + z + u) & 1)) sr = -sr;
     for (int x = threadIdx.x; x < Ox; x += 64) {
         const int ix = x - offx;
         float2 v = make_float2(0.0f, 0.0f);
@@ -669,22 +703,27 @@ __global__ __launch_bounds__(64 * kRowsPerBlock) void pad_deapod_kernel(const fl
 // a 64-thread row per output line.
 __global__ __launch_bounds__(64 * kRowsPerBlock) void crop_deapod_kernel(const float2* in, float2* out,
                                                                          long long nb, GridGeom g, PadCrop pc) {
-    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2];
+    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2], Ov = g.nos[3];
     const int Nx = pc.in_os ? Ox : pc.N[0], Ny = pc.in_os ? Oy : pc.N[1], Nz = pc.in_os ? Oz : pc.N[2];
+    const int Nv = pc.in_os ? Ov : pc.N[3];
     const long long R = (long long)blockIdx.x * kRowsPerBlock + threadIdx.y;
-    if (R >= nb * Ny * Nz) return;
+    if (R >= nb * Ny * Nz * Nv) return;
     const int iy = (int)(R % Ny);
     const long long t = R / Ny;
     const int iz = (int)(t % Nz);
-    const long long b = t / Nz;
+    const long long t2 = t / Nz;
+    const int iv = (int)(t2 % Nv);
+    const long long b = t2 / Nv;
     const int y = pc.in_os ? iy : iy + pc.off[1], z = pc.in_os ? iz : iz + pc.off[2];
+    const int u = pc.in_os ? iv : iv + pc.off[3];
     const int offx = pc.in_os ? 0 : pc.off[0];
-    const float2* src = in + ((b * Oz + z) * Oy + y) * (long long)Ox + offx;
+    const float2* src = in + (((b * Ov + u) * Oz + z) * Oy + y) * (long long)Ox + offx;
     float2* dst = out + R * Nx;
     float sr = pc.scale;
     if (g.D >= 2) sr *= pc.filt[1][y];
     if (g.D >= 3) sr *= pc.filt[2][z];
-    if (pc.sign && ((y + z) & 1)) sr = -sr;
+    if (g.D >= 4) sr *= pc.filt[3][u];
+    if (pc.sign && ((y + z + u) & 1)) sr = -sr;
     for (int ix = threadIdx.x; ix < Nx; ix += 64) {
         const int x = ix + offx;
         float2 v = src[ix];

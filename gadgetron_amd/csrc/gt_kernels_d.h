@@ -19,15 +19,14 @@ constexpr float kBinMargin = 1.0e-3f;
 // p = (k + 1/2) N_os in double, with a float copy for the binning; out-of-range
 // or non-finite input raises the flag.
 __global__ void scale_traj_d_kernel(const double* __restrict__ traj, double* __restrict__ out,
-                                    float* __restrict__ outf, long long n, int D, double nos0, double nos1,
-                                    double nos2, int* __restrict__ flag) {
+                                    float* __restrict__ outf, long long n, int D, GridGeom g,
+                                    int* __restrict__ flag) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int d = (int)(i % D);
     const double k = traj[i];
-    const double nos = d == 0 ? nos0 : (d == 1 ? nos1 : nos2);
     if (!(k >= -0.5 && k <= 0.5)) flag[0] = 1;
-    const double v = (k + 0.5) * nos;
+    const double v = (k + 0.5) * g.nos_d[d];
     out[i] = v;
     outf[i] = (float)v;
 }
@@ -62,11 +61,12 @@ __global__ __launch_bounds__(256) void c2nc_gather_d_kernel(const double2* __res
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsamp) return;
     const int f = (int)(j / M);
-    double wx[KW], wy[KW], wz[KW];
-    int cx[KW], cy[KW], cz[KW];
+    double wx[KW], wy[KW], wz[KW], wv[KW];
+    int cx[KW], cy[KW], cz[KW], cv[KW];
     window1d_d<KW>(p[j * D + 0], g, 0, sign != 0, wx, cx);
     if (D >= 2) window1d_d<KW>(p[j * D + 1], g, 1, sign != 0, wy, cy);
     if (D >= 3) window1d_d<KW>(p[j * D + 2], g, 2, sign != 0, wz, cz);
+    if (D >= 4) window1d_d<KW>(p[j * D + 3], g, 3, sign != 0, wv, cv);
     const long long G = g.G;
     const long long FG = (long long)F * G;
     const double2* gb = grid + (long long)f * G;
@@ -77,8 +77,10 @@ __global__ __launch_bounds__(256) void c2nc_gather_d_kernel(const double2* __res
             ar[q] = 0.0;
             ai[q] = 0.0;
         }
+        const int nv = D >= 4 ? KW : 1;
         const int nz = D >= 3 ? KW : 1;
         const int ny = D >= 2 ? KW : 1;
+        for (int iv = 0; iv < nv; iv++)
         for (int iz = 0; iz < nz; iz++) {
             for (int iy = 0; iy < ny; iy++) {
 #pragma unroll
@@ -91,9 +93,13 @@ __global__ __launch_bounds__(256) void c2nc_gather_d_kernel(const double2* __res
                     } else if (D == 2) {
                         w = wx[ix] * wy[iy];
                         cell = cx[ix] + (long long)g.nos[0] * cy[iy];
-                    } else {
+                    } else if (D == 3) {
                         w = (wx[ix] * wy[iy]) * wz[iz];
                         cell = cx[ix] + (long long)g.nos[0] * (cy[iy] + (long long)g.nos[1] * cz[iz]);
+                    } else {
+                        w = ((wx[ix] * wy[iy]) * wz[iz]) * wv[iv];
+                        cell = cx[ix] + (long long)g.nos[0] *
+                                            (cy[iy] + (long long)g.nos[1] * (cz[iz] + (long long)g.nos[2] * cv[iv]));
                     }
                     if (w != 0.0) {
 #pragma unroll
@@ -152,7 +158,7 @@ __global__ __launch_bounds__(64) void nc2c_tile_d_kernel(const double2* __restri
     __syncthreads();
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     int t = (int)(it.key % (uint32_t)g.ntiles_total);
-    int org[3] = {0, 0, 0};
+    int org[4] = {0, 0, 0, 0};
     for (int d = 0; d < D; d++) {
         org[d] = (t % g.ntiles[d]) * g.tile[d];
         t /= g.ntiles[d];
@@ -162,11 +168,12 @@ __global__ __launch_bounds__(64) void nc2c_tile_d_kernel(const double2* __restri
         const uint32_t e = e0 + threadIdx.x;
         const bool active = e < it.end;
         const uint32_t j = active ? tile_samples[e] : 0u;
-        double wx[KW], wy[KW], wz[KW];
-        int cx[KW], cy[KW], cz[KW];
+        double wx[KW], wy[KW], wz[KW], wv[KW];
+        int cx[KW], cy[KW], cz[KW], cv[KW];
         window1d_d<KW>(p[(long long)j * D + 0], g, 0, false, wx, cx);
         if (D >= 2) window1d_d<KW>(p[(long long)j * D + 1], g, 1, false, wy, cy);
         if (D >= 3) window1d_d<KW>(p[(long long)j * D + 2], g, 2, false, wz, cz);
+        if (D >= 4) window1d_d<KW>(p[(long long)j * D + 3], g, 3, false, wv, cv);
 #pragma unroll
         for (int i = 0; i < KW; i++) {
             int l = cx[i] - org[0];
@@ -179,6 +186,10 @@ __global__ __launch_bounds__(64) void nc2c_tile_d_kernel(const double2* __restri
                 l = cz[i] - org[2];
                 cz[i] = (wz[i] != 0.0 && l >= 0 && l < g.tile[2]) ? l : -1;
             }
+            if (D >= 4) {
+                l = cv[i] - org[3];
+                cv[i] = (wv[i] != 0.0 && l >= 0 && l < g.tile[3]) ? l : -1;
+            }
         }
         for (int q = 0; q < ncc; q++) {
             const long long sidx = (long long)(c0 + q) * FM + j;
@@ -190,9 +201,12 @@ __global__ __launch_bounds__(64) void nc2c_tile_d_kernel(const double2* __restri
             }
             double* are = acc_re + q * TC;
             double* aim = acc_im + q * TC;
+            const int nv = D >= 4 ? KW : 1;
             const int nz = D >= 3 ? KW : 1;
             const int ny = D >= 2 ? KW : 1;
+            for (int iv = 0; iv < nv; iv++)
             for (int iz = 0; iz < nz; iz++) {
+                if (D >= 4 && cv[iv] < 0) break;  // the same for every iz
                 if (D >= 3 && cz[iz] < 0) continue;
                 for (int iy = 0; iy < ny; iy++) {
                     if (D >= 2 && cy[iy] < 0) continue;
@@ -207,9 +221,12 @@ __global__ __launch_bounds__(64) void nc2c_tile_d_kernel(const double2* __restri
                         } else if (D == 2) {
                             w = wx[ix] * wy[iy];
                             lc = cx[ix] + g.tile[0] * cy[iy];
-                        } else {
+                        } else if (D == 3) {
                             w = (wx[ix] * wy[iy]) * wz[iz];
                             lc = cx[ix] + g.tile[0] * (cy[iy] + g.tile[1] * cz[iz]);
+                        } else {
+                            w = ((wx[ix] * wy[iy]) * wz[iz]) * wv[iv];
+                            lc = cx[ix] + g.tile[0] * (cy[iy] + g.tile[1] * (cz[iz] + g.tile[2] * cv[iv]));
                         }
                         atomicAdd(&are[lc], s.x * w);
                         atomicAdd(&aim[lc], s.y * w);
@@ -228,7 +245,7 @@ __global__ __launch_bounds__(64) void nc2c_tile_d_kernel(const double2* __restri
             partials[((long long)it.partial * C + (c0 + q)) * TC + lc] = v;
             continue;
         }
-        int gc[3] = {0, 0, 0};
+        int gc[4] = {0, 0, 0, 0};
         bool inside = true;
         int par = 0;
         for (int d = 0; d < D; d++) {
@@ -242,7 +259,8 @@ __global__ __launch_bounds__(64) void nc2c_tile_d_kernel(const double2* __restri
             v.x = -v.x;
             v.y = -v.y;
         }
-        const long long cell = gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * gc[2]);
+        const long long cell =
+            gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * (gc[2] + (long long)g.nos[2] * gc[3]));
         const long long o = ((long long)(c0 + q) * F + f) * G + cell;
         if (accumulate) {
             const double2 old = grid[o];
@@ -262,7 +280,7 @@ __global__ __launch_bounds__(256) void nc2c_reduce_d_kernel(const double2* __res
     const int TC = g.tile_cells;
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     int t = (int)(it.key % (uint32_t)g.ntiles_total);
-    int org[3] = {0, 0, 0};
+    int org[4] = {0, 0, 0, 0};
     for (int d = 0; d < g.D; d++) {
         org[d] = (t % g.ntiles[d]) * g.tile[d];
         t /= g.ntiles[d];
@@ -278,7 +296,7 @@ __global__ __launch_bounds__(256) void nc2c_reduce_d_kernel(const double2* __res
             v.y += a.y;
         }
         int lc = lc0;
-        int gc[3] = {0, 0, 0};
+        int gc[4] = {0, 0, 0, 0};
         bool inside = true;
         int par = 0;
         for (int d = 0; d < g.D; d++) {
@@ -292,7 +310,8 @@ __global__ __launch_bounds__(256) void nc2c_reduce_d_kernel(const double2* __res
             v.x = -v.x;
             v.y = -v.y;
         }
-        const long long cell = gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * gc[2]);
+        const long long cell =
+            gc[0] + (long long)g.nos[0] * (gc[1] + (long long)g.nos[1] * (gc[2] + (long long)g.nos[2] * gc[3]));
         const long long o = ((long long)q * F + f) * G + cell;
         if (accumulate) {
             const double2 old = grid[o];
@@ -304,9 +323,9 @@ __global__ __launch_bounds__(256) void nc2c_reduce_d_kernel(const double2* __res
 }
 
 struct PadCropD {
-    int N[3];
-    int off[3];
-    const double* filt[3];
+    int N[4];
+    int off[4];
+    const double* filt[4];
     double scale;
     int sign;
     int in_os;
@@ -317,25 +336,30 @@ __global__ __launch_bounds__(64 * kRowsPerBlock) void pad_deapod_d_kernel(const 
                                                                           double2* __restrict__ grid, long long nb,
                                                                           GridGeom g, PadCropD pc) {
     const long long R = (long long)blockIdx.x * kRowsPerBlock + threadIdx.y;
-    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2];
-    if (R >= nb * Oy * Oz) return;
+    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2], Ov = g.nos[3];
+    if (R >= nb * Oy * Oz * Ov) return;
     const int y = (int)(R % Oy);
     const long long t = R / Oy;
     const int z = (int)(t % Oz);
-    const long long b = t / Oz;
+    const long long t2 = t / Oz;
+    const int u = (int)(t2 % Ov);
+    const long long b = t2 / Ov;
     double2* out = grid + R * Ox;
     const int iy = pc.in_os ? y : y - pc.off[1], iz = pc.in_os ? z : z - pc.off[2];
+    const int iv = pc.in_os ? u : u - pc.off[3];
     const int Nx = pc.in_os ? Ox : pc.N[0], Ny = pc.in_os ? Oy : pc.N[1], Nz = pc.in_os ? Oz : pc.N[2];
+    const int Nv = pc.in_os ? Ov : pc.N[3];
     const int offx = pc.in_os ? 0 : pc.off[0];
-    if (iy < 0 || iy >= Ny || iz < 0 || iz >= Nz) {
+    if (iy < 0 || iy >= Ny || iz < 0 || iz >= Nz || iv < 0 || iv >= Nv) {
         for (int x = threadIdx.x; x < Ox; x += 64) out[x] = make_double2(0.0, 0.0);
         return;
     }
-    const double2* in = img + ((b * Nz + iz) * Ny + iy) * Nx;
+    const double2* in = img + (((b * Nv + iv) * Nz + iz) * Ny + iy) * Nx;
     double sr = pc.scale;
     if (g.D >= 2) sr *= pc.filt[1][y];
     if (g.D >= 3) sr *= pc.filt[2][z];
-    if (pc.sign && ((y + z) & 1)) sr = -sr;
+    if (g.D >= 4) sr *= pc.filt[3][u];
+    if (pc.sign && ((y + z + u) & 1)) sr = -sr;
     for (int x = threadIdx.x; x < Ox; x += 64) {
         const int ix = x - offx;
         double2 v = make_double2(0.0, 0.0);
@@ -352,22 +376,27 @@ __global__ __launch_bounds__(64 * kRowsPerBlock) void pad_deapod_d_kernel(const 
 // Crop (or copy) + deapodize + centring sign + scale: a 64-thread row per output line.
 __global__ __launch_bounds__(64 * kRowsPerBlock) void crop_deapod_d_kernel(const double2* in, double2* out,
                                                                            long long nb, GridGeom g, PadCropD pc) {
-    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2];
+    const int Ox = g.nos[0], Oy = g.nos[1], Oz = g.nos[2], Ov = g.nos[3];
     const int Nx = pc.in_os ? Ox : pc.N[0], Ny = pc.in_os ? Oy : pc.N[1], Nz = pc.in_os ? Oz : pc.N[2];
+    const int Nv = pc.in_os ? Ov : pc.N[3];
     const long long R = (long long)blockIdx.x * kRowsPerBlock + threadIdx.y;
-    if (R >= nb * Ny * Nz) return;
+    if (R >= nb * Ny * Nz * Nv) return;
     const int iy = (int)(R % Ny);
     const long long t = R / Ny;
     const int iz = (int)(t % Nz);
-    const long long b = t / Nz;
+    const long long t2 = t / Nz;
+    const int iv = (int)(t2 % Nv);
+    const long long b = t2 / Nv;
     const int y = pc.in_os ? iy : iy + pc.off[1], z = pc.in_os ? iz : iz + pc.off[2];
+    const int u = pc.in_os ? iv : iv + pc.off[3];
     const int offx = pc.in_os ? 0 : pc.off[0];
-    const double2* src = in + ((b * Oz + z) * Oy + y) * (long long)Ox + offx;
+    const double2* src = in + (((b * Ov + u) * Oz + z) * Oy + y) * (long long)Ox + offx;
     double2* dst = out + R * Nx;
     double sr = pc.scale;
     if (g.D >= 2) sr *= pc.filt[1][y];
     if (g.D >= 3) sr *= pc.filt[2][z];
-    if (pc.sign && ((y + z) & 1)) sr = -sr;
+    if (g.D >= 4) sr *= pc.filt[3][u];
+    if (pc.sign && ((y + z + u) & 1)) sr = -sr;
     for (int ix = threadIdx.x; ix < Nx; ix += 64) {
         const int x = ix + offx;
         double2 v = src[ix];

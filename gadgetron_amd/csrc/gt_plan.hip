@@ -57,6 +57,7 @@ std::once_flag g_rocfft_once;
 
 struct FFTPlan {
     rocfft_plan fwd = nullptr, bwd = nullptr;
+    rocfft_plan fwd2 = nullptr, bwd2 = nullptr;  // D = 4: the 1-D pass along the 4th dimension
     rocfft_execution_info info = nullptr;
     DevBuf work;
 };
@@ -138,25 +139,25 @@ struct Binning {
 
 struct gt_nufft_plan {
     int D = 2;
-    size_t N[3] = {1, 1, 1}, Nos[3] = {1, 1, 1};
+    size_t N[4] = {1, 1, 1, 1}, Nos[4] = {1, 1, 1, 1};
     double W = 0;
-    float Wf = 0, radius = 0, one_over_W = 0, beta[3] = {0, 0, 0};
+    float Wf = 0, radius = 0, one_over_W = 0, beta[4] = {0, 0, 0, 0};
     int conv_type = GT_CONV_STANDARD;
     int device = 0;
     int cus = 0;  // compute units of the device
     int KW = 0;
-    int tile_req[3] = {0, 0, 0};
+    int tile_req[4] = {0, 0, 0, 0};
     int max_items_req = 0;
     bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
     bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
     bool fast3d = false;     // the 3D hot path (register-tile NC2C, staged C2NC)
     bool dbl = false;        // GT_DOUBLE: the generic path in double precision
-    double radius_d = 0, one_over_W_d = 0, beta_d[3] = {0, 0, 0}, nos_d[3] = {1, 1, 1};
+    double radius_d = 0, one_over_W_d = 0, beta_d[4] = {0, 0, 0, 0}, nos_d[4] = {1, 1, 1, 1};
 
     gt::GridGeom geom{};  // geometry for the generic kernels
-    std::vector<float> filt_host[3];
-    DevBuf filt[3];
-    DevBuf filt_d[3];   // GT_DOUBLE: the filters in double
+    std::vector<float> filt_host[4];
+    DevBuf filt[4];
+    DevBuf filt_d[4];   // GT_DOUBLE: the filters in double
     DevBuf twiddle[2];  // exp(-2 pi i m / N_os[d]), m < N_os[d] (fused FFT)
 
     // preprocess state
@@ -205,6 +206,8 @@ struct gt_nufft_plan {
         for (auto& kv : ffts) {
             if (kv.second.fwd) rocfft_plan_destroy(kv.second.fwd);
             if (kv.second.bwd) rocfft_plan_destroy(kv.second.bwd);
+            if (kv.second.fwd2) rocfft_plan_destroy(kv.second.fwd2);
+            if (kv.second.bwd2) rocfft_plan_destroy(kv.second.bwd2);
             if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
             kv.second.work.release();
         }
@@ -309,11 +312,13 @@ std::vector<double> deapod_1d_d(size_t nos, double radius, double one_over_W, do
 void set_device(const gt_nufft_plan* p) { GT_HIP(hipSetDevice(p->device)); }
 
 void finish_fft_plan(FFTPlan& fp) {
-    size_t w1 = 0, w2 = 0;
+    size_t w1 = 0, w2 = 0, w3 = 0, w4 = 0;
     GT_FFT(rocfft_plan_get_work_buffer_size(fp.fwd, &w1));
     GT_FFT(rocfft_plan_get_work_buffer_size(fp.bwd, &w2));
+    if (fp.fwd2) GT_FFT(rocfft_plan_get_work_buffer_size(fp.fwd2, &w3));
+    if (fp.bwd2) GT_FFT(rocfft_plan_get_work_buffer_size(fp.bwd2, &w4));
     GT_FFT(rocfft_execution_info_create(&fp.info));
-    const size_t w = std::max(w1, w2);
+    const size_t w = std::max(std::max(w1, w2), std::max(w3, w4));
     if (w) {
         fp.work.reserve(w);
         GT_FFT(rocfft_execution_info_set_work_buffer(fp.info, fp.work.ptr, w));
@@ -330,7 +335,32 @@ FFTPlan& get_fft(gt_nufft_plan* p, int kind, size_t batch) {
     if (it != p->ffts.end()) return it->second;
     std::call_once(g_rocfft_once, [] { rocfft_setup(); });
     FFTPlan fp;
-    if (kind == FFT_IMAGE) {
+    if (kind == FFT_IMAGE && p->D == 4) {
+        // rocFFT transforms have rank <= 3: the 3-D transforms of every volume
+        // (batch * N_os[3] of them), then 1-D transforms along the 4th dimension
+        // (stride V = N_os[0] N_os[1] N_os[2], V of them per image)
+        const auto prec = p->dbl ? rocfft_precision_double : rocfft_precision_single;
+        size_t lengths[3] = {p->Nos[0], p->Nos[1], p->Nos[2]};
+        GT_FFT(rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, prec, 3,
+                                  lengths, batch * p->Nos[3], nullptr));
+        GT_FFT(rocfft_plan_create(&fp.bwd, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, prec, 3,
+                                  lengths, batch * p->Nos[3], nullptr));
+        rocfft_plan_description desc = nullptr;
+        GT_FFT(rocfft_plan_description_create(&desc));
+        size_t len = p->Nos[3], stride = p->Nos[0] * p->Nos[1] * p->Nos[2];
+        const size_t dist = 1;
+        rocfft_status st = rocfft_plan_description_set_data_layout(
+            desc, rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved, nullptr, nullptr, 1,
+            &stride, dist, 1, &stride, dist);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&fp.fwd2, rocfft_placement_inplace, rocfft_transform_type_complex_forward, prec, 1,
+                                    &len, stride, desc);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&fp.bwd2, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, prec, 1,
+                                    &len, stride, desc);
+        rocfft_plan_description_destroy(desc);
+        GT_FFT(st);
+    } else if (kind == FFT_IMAGE) {
         size_t lengths[3] = {p->Nos[0], p->Nos[1], p->Nos[2]};  // fastest dimension first
         GT_FFT(rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward,
                                   (p->dbl ? rocfft_precision_double : rocfft_precision_single), p->D, lengths, batch, nullptr));
@@ -391,6 +421,13 @@ void exec_fft(gt_nufft_plan* p, int kind, size_t batch, void* data, bool forward
     GT_FFT(rocfft_execution_info_set_stream(fp.info, s));
     void* buf[1] = {data};
     GT_FFT(rocfft_execute(forward ? fp.fwd : fp.bwd, buf, nullptr, fp.info));
+    if (fp.fwd2) {
+        const size_t esz = p->dbl ? sizeof(double2) : sizeof(float2);
+        for (size_t b = 0; b < batch; b++) {
+            void* img[1] = {static_cast<char*>(data) + b * p->G() * esz};
+            GT_FFT(rocfft_execute(forward ? fp.fwd2 : fp.bwd2, img, nullptr, fp.info));
+        }
+    }
 }
 
 void run_fft(gt_nufft_plan* p, void* data, size_t batch, bool forward, hipStream_t s) {
@@ -411,7 +448,7 @@ float centring_global_sign(const gt_nufft_plan* p) {
 
 gt::PadCrop padcrop(const gt_nufft_plan* p, bool pad_side, bool in_os, float scale, int sign) {
     gt::PadCrop pc{};
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 4; d++) {
         pc.N[d] = d < p->D ? int(p->N[d]) : 1;
         if (d < p->D) {
             // pad offset N_os/2 - N/2 (cuNDArray_utils.cu:283-286), crop offset (N_os-N)>>1 (NFFT.hpp:178)
@@ -455,6 +492,20 @@ void dispatch(int D, int KW, A&&... a) {
     GT_CASES(2)
     GT_CASES(3)
 #undef GT_CASES
+#undef GT_CASE
+    throw gt_error(GT_ERR_UNSUPPORTED, "unsupported (D, window) combination");
+}
+
+// dispatch for the kernels that also take D = 4 (the generic gridding kernels)
+template <template <int, int> class L, class... A>
+void dispatch4(int D, int KW, A&&... a) {
+    if (D <= 3) {
+        dispatch<L>(D, KW, std::forward<A>(a)...);
+        return;
+    }
+#define GT_CASE(k) \
+    if (D == 4 && KW == k) { L<4, k>::run(std::forward<A>(a)...); return; }
+    GT_CASE(2) GT_CASE(3) GT_CASE(4) GT_CASE(5) GT_CASE(6) GT_CASE(7) GT_CASE(8) GT_CASE(9)
 #undef GT_CASE
     throw gt_error(GT_ERR_UNSUPPORTED, "unsupported (D, window) combination");
 }
@@ -679,7 +730,7 @@ gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile, bool tile_covers
     g.G = (long long)p->G();
     g.ntiles_total = 1;
     g.tile_cells = 1;
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 4; d++) {
         if (d < p->D) {
             g.nos[d] = int(p->Nos[d]);
             g.nos_f[d] = float(unsigned(p->Nos[d]));
@@ -705,12 +756,16 @@ gt::GridGeom make_geom(const gt_nufft_plan* p, const int* tile, bool tile_covers
 }
 
 void setup_geometry(gt_nufft_plan* p) {
-    int tile[3];
-    for (int d = 0; d < 3; d++) {
-        const int def = p->D == 1 ? 256 : (p->D == 2 ? 16 : 8);
+    int tile[4];
+    for (int d = 0; d < 4; d++) {
+        // D = 4: tiles as wide as the window (a tile of KW^4 cells per wavefront)
+        const int def = p->D == 1 ? 256 : (p->D == 2 ? 16 : (p->D == 3 ? 8 : 1));
         tile[d] = p->tile_req[d] > 0 ? p->tile_req[d] : def;
     }
     p->geom = make_geom(p, tile);
+    // the generic NC2C holds at least one coil's tile of re/im accumulators in LDS
+    GT_REQUIRE(size_t(2) * (p->dbl ? sizeof(double) : sizeof(float)) * size_t(p->geom.tile_cells) <= 65536,
+               GT_ERR_UNSUPPORTED, "the gridding tile does not fit in LDS (4-D double plans take windows of <= 8 cells)");
     // 2D hot path: STANDARD gridding, a window of 3..7 cells, 8 x 8 tiles
     // (the default; any other requested tile selects the generic kernels) and
     // a grid of at least tile + window cells per dimension.
@@ -1065,7 +1120,7 @@ void conv_c2nc(gt_nufft_plan* p, const void* grid, const gt::Layout& L, void* ou
         dispatch_fast<LaunchFast3C2NC>(p->KW, (const float*)grid, (float*)out, p, int(C), L, sign, dcw, dl,
                                        accumulate ? 1 : 0, s);
     } else {
-        dispatch<LaunchC2NC>(p->D, p->KW, (const float2*)grid, (float2*)out, p->traj.as<float>(), int(p->M),
+        dispatch4<LaunchC2NC>(p->D, p->KW, (const float2*)grid, (float2*)out, p->traj.as<float>(), int(p->M),
                              int(p->F), int(C), p->geom, sign, dcw, dl, accumulate ? 1 : 0, s);
     }
     GT_HIP(hipGetLastError());
@@ -1098,7 +1153,7 @@ void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layo
     const long long dl = (long long)std::max<size_t>(dcw_len, 1);
     if (p->conv_type == GT_CONV_ATOMIC) {
         if (!accumulate) GT_HIP(hipMemsetAsync(grid, 0, p->G() * p->F * C * sizeof(float2), s));
-        dispatch<LaunchNC2CAtomic>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->traj.as<float>(),
+        dispatch4<LaunchNC2CAtomic>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->traj.as<float>(),
                                    int(p->M), int(p->F), int(C), p->geom, sign, dcw, dl, s);
         GT_HIP(hipGetLastError());
         return;
@@ -1130,7 +1185,7 @@ void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layo
         return;
     }
     const int CC = coil_chunk(p);
-    dispatch<LaunchNC2CTile>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->partials.as<float2>(),
+    dispatch4<LaunchNC2CTile>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->partials.as<float2>(),
                              p->traj.as<float>(), b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), b.n_items,
                              int(p->M), int(p->F), int(C), CC, b.geom, sign, dcw, dl, accumulate ? 1 : 0, s);
     GT_HIP(hipGetLastError());
@@ -1168,7 +1223,7 @@ void crop_deapod(gt_nufft_plan* p, const void* grid, void* img, size_t nb, bool 
 // ---- double-precision plans: the generic path in double ---------------------
 gt::PadCropD padcrop_d(const gt_nufft_plan* p, bool pad_side, bool in_os, double scale, int sign) {
     gt::PadCropD pc{};
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 4; d++) {
         pc.N[d] = d < p->D ? int(p->N[d]) : 1;
         if (d < p->D) {
             pc.off[d] = pad_side ? int(p->Nos[d] / 2 - p->N[d] / 2) : int((p->Nos[d] - p->N[d]) >> 1);
@@ -1210,7 +1265,7 @@ void conv_c2nc_d(gt_nufft_plan* p, const void* grid, void* out, size_t C, int si
                  size_t dcw_len, bool accumulate, hipStream_t s) {
     GT_REQUIRE(p->prep_c2nc, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for C2NC");
     StageTimer timer(p, GT_STAGE_C2NC, s);
-    dispatch<LaunchC2NCD>(p->D, p->KW, (const double2*)grid, (double2*)out, p->traj_d.as<double>(), int(p->M),
+    dispatch4<LaunchC2NCD>(p->D, p->KW, (const double2*)grid, (double2*)out, p->traj_d.as<double>(), int(p->M),
                           int(p->F), int(C), p->geom, sign, dcw, (long long)std::max<size_t>(dcw_len, 1),
                           accumulate ? 1 : 0, s);
     GT_HIP(hipGetLastError());
@@ -1226,7 +1281,7 @@ void conv_nc2c_d(gt_nufft_plan* p, const void* samples, void* grid, size_t C, in
     // LDS budget per one-wave workgroup: 16 KiB of re/im accumulators
     const int CC = std::max(1, std::min(16384 / (16 * TC), 32));
     const long long dl = (long long)std::max<size_t>(dcw_len, 1);
-    dispatch<LaunchNC2CTileD>(p->D, p->KW, (const double2*)samples, (double2*)grid, p->partials.as<double2>(),
+    dispatch4<LaunchNC2CTileD>(p->D, p->KW, (const double2*)samples, (double2*)grid, p->partials.as<double2>(),
                               p->traj_d.as<double>(), b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), b.n_items,
                               int(p->M), int(p->F), int(C), CC, b.geom, sign, dcw, dl, accumulate ? 1 : 0, s);
     GT_HIP(hipGetLastError());
@@ -1513,7 +1568,7 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
     return guarded([&] {
         GT_REQUIRE(out && d, GT_ERR_INVALID_ARGUMENT, "null plan or descriptor");
         *out = nullptr;
-        GT_REQUIRE(d->D >= 1 && d->D <= 3, GT_ERR_UNSUPPORTED, "D must be 1, 2 or 3");
+        GT_REQUIRE(d->D >= 1 && d->D <= 4, GT_ERR_UNSUPPORTED, "D must be 1 to 4");
         GT_REQUIRE(d->precision == GT_FLOAT || d->precision == GT_DOUBLE, GT_ERR_UNSUPPORTED,
                    "precision must be GT_FLOAT or GT_DOUBLE");
         // NFFT<cuNDArray, double, D>::make_plan (cuNFFT.cu:219-227)
@@ -1647,12 +1702,11 @@ gt_status gt_nufft_preprocess(gt_nufft_plan* p, const void* traj, size_t M, size
             // traj: double D-vectors; positions in double, a float copy for the binning
             p->traj_d.reserve(size_t(n) * sizeof(double));
             hipLaunchKernelGGL(gt::scale_traj_d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                               (const double*)traj, p->traj_d.as<double>(), p->traj.as<float>(), n, p->D,
-                               p->geom.nos_d[0], p->geom.nos_d[1], p->geom.nos_d[2], p->flag.as<int>());
+                               (const double*)traj, p->traj_d.as<double>(), p->traj.as<float>(), n, p->D, p->geom,
+                               p->flag.as<int>());
         } else {
             hipLaunchKernelGGL(gt::scale_traj_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                               (const float*)traj, p->traj.as<float>(), n, p->D, p->geom.nos_f[0],
-                               p->geom.nos_f[1], p->geom.nos_f[2], p->flag.as<int>());
+                               (const float*)traj, p->traj.as<float>(), n, p->D, p->geom, p->flag.as<int>());
         }
         GT_HIP(hipGetLastError());
         int bad = 0;
@@ -1791,6 +1845,7 @@ gt_status gt_nufft_grad_traj(gt_nufft_plan* p, const void* image, const size_t* 
     return guarded([&] {
         GT_REQUIRE(p && image && grad_samples && grad_traj, GT_ERR_INVALID_ARGUMENT, "null argument");
         GT_REQUIRE(!p->dbl, GT_ERR_UNSUPPORTED, "the trajectory gradient is provided for float plans");
+        GT_REQUIRE(p->D <= 3, GT_ERR_UNSUPPORTED, "the trajectory gradient is provided for D <= 3");
         GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
         GT_REQUIRE(p->M > 0 && p->F > 0, GT_ERR_NOT_PREPROCESSED, "plan has not been preprocessed");
         Shape sh = image_shape(p, dims, ndim, true);

@@ -69,7 +69,7 @@ typedef struct gt_nufft_plan gt_nufft_plan;
  * Both use the cuNFFT deapodization convention (unscaled filter FFT,
  * cuNFFT.cu:170-202), which equals hoNFFT_plan(N, alpha, W) (hoNFFT.cpp:175-197). */
 typedef struct gt_nufft_desc {
-    int D;               /* 1, 2 or 3 */
+    int D;               /* 1 to 4 (4: no trajectory gradient, no SENSE) */
     int precision;       /* GT_FLOAT, or GT_DOUBLE (NFFT<cuNDArray, double, D>, cuNFFT.cu:219-227):
                             STANDARD convolution only, the generic kernels in double;
                             trajectories are double D-vectors, samples and images
@@ -131,8 +131,9 @@ gt_status gt_nufft_get_matrix_size_os(const gt_nufft_plan* plan, size_t* N_os);
 gt_status gt_nufft_get_W(const gt_nufft_plan* plan, double* W);
 gt_status gt_nufft_get_num_samples_frames(const gt_nufft_plan* plan, size_t* M, size_t* F);
 
-/* Tuning knobs of the STANDARD gridding (tile edge per dim, samples per work
- * item).  Values <= 0 keep the defaults.  Takes effect at the next preprocess. */
+/* Tuning knobs of the STANDARD gridding (tile edge of dims 0..2, samples per
+ * work item).  Values <= 0 keep the defaults; a 4-D plan's 4th dimension keeps
+ * tiles as wide as the window.  Takes effect at the next preprocess. */
 gt_status gt_nufft_set_tiling(gt_nufft_plan* plan, const int* tile, int max_samples_per_item);
 
 /* Stage timing with HIP events recorded on the caller's stream around each

@@ -92,7 +92,7 @@ std::vector<size_t> dims_of(const A& a) {
 // DCW array of a double plan holds doubles).
 template <class REAL, unsigned int D>
 class cuNFFT_plan {
-    static_assert(D >= 1 && D <= 3, "D must be 1, 2 or 3");
+    static_assert(D >= 1 && D <= 4, "D must be 1 to 4");
 
 public:
     using vec = std::array<size_t, D>;

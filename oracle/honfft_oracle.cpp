@@ -105,9 +105,9 @@ template <class R>
 struct Kernel {
     int D = 2;
     R width = 0, radius = 0, width_inv = 0;
-    R nos[3] = {0, 0, 0};
-    R os_factor[3] = {0, 0, 0};
-    R beta[3] = {0, 0, 0};
+    R nos[4] = {0, 0, 0, 0};
+    R os_factor[4] = {0, 0, 0, 0};
+    R beta[4] = {0, 0, 0, 0};
 
     // ConvolutionKernel.hpp:287-303
     void compute_beta() {
@@ -336,7 +336,7 @@ template <class R>
 void window_rec(const Kernel<R>& k, const R* point, const size_t* nos, int D, int dim,
                 R* image_point, size_t index, std::vector<uint32_t>& idx, std::vector<R>& wts) {
     if (dim < 0) {
-        R delta[3];
+        R delta[4];
         for (int d = 0; d < D; d++) delta[d] = std::abs(image_point[d] - point[d]);
         idx.push_back(uint32_t(index));
         wts.push_back(k.get(delta));
@@ -365,7 +365,7 @@ ConvMatrix<R> make_conv_matrix(const R* traj_scaled, size_t M, const size_t* nos
     m.weights.resize(M);
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)M; i++) {
-        R ip[3] = {0, 0, 0};
+        R ip[4] = {0, 0, 0, 0};
         window_rec<R>(k, traj_scaled + size_t(i) * D, nos, D, D - 1, ip, 0, m.indices[i], m.weights[i]);
     }
     return m;
@@ -408,7 +408,7 @@ template <class R>
 struct Plan : PlanBase {
     using C = std::complex<R>;
     int D = 2;
-    size_t N[3] = {1, 1, 1}, Nos[3] = {1, 1, 1};
+    size_t N[4] = {1, 1, 1, 1}, Nos[4] = {1, 1, 1, 1};
     R W = 0;
     Kernel<R> kernel;
     std::vector<C> deapod_ifft, deapod_fft;  // reciprocal filters, G elements
@@ -425,7 +425,7 @@ struct Plan : PlanBase {
         for (size_t idx = 0; idx < g; idx++) {
             size_t r = idx;
             R w = R(1);
-            R ws[3] = {R(1), R(1), R(1)};
+            R ws[4] = {R(1), R(1), R(1), R(1)};
             for (int d = 0; d < D; d++) {
                 size_t c = r % Nos[d];
                 r /= Nos[d];
@@ -514,7 +514,7 @@ struct Plan : PlanBase {
     void pad(const C* in, C* out, size_t nbatches) const {
         const size_t g = G(), n = Nimg();
         std::fill(out, out + g * nbatches, C(0));
-        size_t off[3];
+        size_t off[4];
         for (int d = 0; d < D; d++) off[d] = Nos[d] / 2 - N[d] / 2;
         for (size_t b = 0; b < nbatches; b++)
             for (size_t idx = 0; idx < n; idx++) {
@@ -531,7 +531,7 @@ struct Plan : PlanBase {
     // NFFT.hpp:155,178: crop offset (N_os - N) >> 1
     void crop(const C* in, C* out, size_t nbatches) const {
         const size_t g = G(), n = Nimg();
-        size_t off[3];
+        size_t off[4];
         for (int d = 0; d < D; d++) off[d] = (Nos[d] - N[d]) >> 1;
         for (size_t b = 0; b < nbatches; b++)
             for (size_t idx = 0; idx < n; idx++) {
@@ -617,7 +617,7 @@ struct Plan : PlanBase {
 
 std::unique_ptr<PlanBase> make_plan(int D, int precision, const size_t* N, const size_t* Nos,
                                     double W, int ctor, double alpha) {
-    if (D < 1 || D > 3) throw std::runtime_error("D must be 1, 2 or 3");
+    if (D < 1 || D > 4) throw std::runtime_error("D must be 1 to 4");
     auto fill = [&](auto* p) {
         using R = std::remove_reference_t<decltype(p->W)>;
         p->D = D;
@@ -918,7 +918,7 @@ int gto_ndft(int D, const size_t* N, const double* traj, size_t M, const void* i
 #pragma omp parallel for schedule(static)
             for (long long idx = 0; idx < (long long)n; idx++) {
                 size_t r = size_t(idx);
-                size_t co[3] = {0, 0, 0};
+                size_t co[4] = {0, 0, 0, 0};
                 for (int d = 0; d < D; d++) { co[d] = r % N[d]; r /= N[d]; }
                 for (size_t b = 0; b < nb; b++) {
                     C acc(0);

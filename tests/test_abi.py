@@ -50,7 +50,7 @@ def _desc(**kw):
 @pytest.mark.parametrize("kw,status", [
     (dict(W=1.5), 1),                       # width < 1.8
     (dict(W=float("nan")), 1),
-    (dict(D=4), 5),
+    (dict(D=5), 5),
     (dict(D=0), 5),
     (dict(precision=1, conv_type=1), 5),    # double plans are STANDARD only
     (dict(precision=2), 5),                 # no such precision

@@ -192,7 +192,7 @@ def test_nfft_adjoint_identity_and_mult_MH_M():
     assert abs(np.vdot(x, Fx) - np.vdot(By, y)) / abs(np.vdot(x, Fx)) < 1e-12
 
 
-@pytest.mark.parametrize("N", [[16], [16, 12], [8, 12, 4]])
+@pytest.mark.parametrize("N", [[16], [16, 12], [8, 12, 4], [4, 8, 4, 4]])
 def test_nfft_exact_on_grid_frequencies(N):
     """For on-grid positions k = m / N_os the gridding is exact, so FORWARDS_C2NC
     equals NDFT / sqrt(G) to rounding; BACKWARDS_NC2C equals the adjoint NDFT.
@@ -313,3 +313,23 @@ def test_nfft_approximation_error_vs_ndft_small():
     y = p.compute(x, (1, 400), po.FORWARDS_C2NC)
     err = rel_l2(y, po.ndft(k, N, x=x) / np.sqrt(G))
     assert 1e-6 < err < 1e-3
+
+
+def test_4d_adjoint_identity_and_ndft_error():
+    """D = 4 (cuNFFT_impl<float, 4>, cuNFFT.cu:270-283): NC2C is the adjoint of
+    C2NC, and off-grid samples approximate the FP64 NDFT as in 2D."""
+    rng = np.random.default_rng(12)
+    N = [8, 4, 4, 8]  # even N_os in every dimension
+    p = _plan(N)
+    G = float(np.prod(p.Nos))
+    M = 300
+    k = rng.uniform(-0.5, 0.5, (M, 4))
+    p.preprocess(k[None])
+    shape = (2,) + tuple(reversed(N))
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    y = rng.standard_normal((2, M)) + 1j * rng.standard_normal((2, M))
+    Ax = p.compute(x, (2, M), po.FORWARDS_C2NC)
+    AHy = p.compute(y, shape, po.BACKWARDS_NC2C)
+    assert abs(np.vdot(y, Ax) - np.vdot(AHy, x)) / abs(np.vdot(y, Ax)) < 1e-12
+    err = rel_l2(Ax, po.ndft(k, N, x=x) / np.sqrt(G))
+    assert 1e-6 < err < 2e-3
