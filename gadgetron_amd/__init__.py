@@ -11,6 +11,7 @@ from .nfft import (
     NFFT_conv_mode,
     NFFT_fft_mode,
     NFFT_prep_mode,
+    NFFTOperator,
     NFFTPlan,
     device_count,
 )
@@ -19,6 +20,6 @@ from . import b1map, buffer, trajectories
 
 __all__ = [
     "CgBatch", "ConvolutionType", "DeviceArray", "EXPORTED_SYMBOLS", "GTError", "LIB_PATH", "NFFT_comp_mode",
-    "NFFT_conv_mode", "NFFT_fft_mode", "NFFT_prep_mode", "NFFTPlan", "SenseOperator", "cg_sense",
+    "NFFT_conv_mode", "NFFT_fft_mode", "NFFT_prep_mode", "NFFTOperator", "NFFTPlan", "SenseOperator", "cg_sense",
     "b1map", "buffer", "device_count", "lib", "trajectories",
 ]

@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = [
     "gt_memcpy_dtoh",
     "gt_memcpy_dtod",
     "gt_memset",
+    "gt_array_add",
     "gt_stream_synchronize",
     "gt_device_synchronize",
     # include/gt_cgsense.h
@@ -165,6 +166,7 @@ def lib():
         "gt_memcpy_dtoh": [P, P, S, P],
         "gt_memcpy_dtod": [P, P, S, P],
         "gt_memset": [P, I, S, P],
+        "gt_array_add": [P, P, S, I, P],
         "gt_stream_synchronize": [P],
         "gt_device_synchronize": [],
         "gt_sense_create": [PP, P, P, S, P, S],

@@ -734,6 +734,13 @@ __global__ __launch_bounds__(64 * kRowsPerBlock) void crop_deapod_kernel(const f
     }
 }
 
+// y += x, grid-stride (gt_array_add).
+template <class T>
+__global__ __launch_bounds__(256) void add_kernel(T* __restrict__ y, const T* __restrict__ x, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] += x[i];
+}
+
 __global__ __launch_bounds__(256) void sign_scale_kernel(float2* __restrict__ data, long long nb,
                                                          GridGeom g, float scale, int sign) {
     const long long G = g.G;

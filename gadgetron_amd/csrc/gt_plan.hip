@@ -1961,6 +1961,22 @@ gt_status gt_memcpy_dtod(void* dst, const void* src, size_t bytes, gt_stream_t s
 gt_status gt_memset(void* ptr, int value, size_t bytes, gt_stream_t s) {
     return guarded([&] { GT_HIP(hipMemsetAsync(ptr, value, bytes, S(s))); });
 }
+gt_status gt_array_add(void* y, const void* x, size_t n, int precision, gt_stream_t s) {
+    return guarded([&] {
+        GT_REQUIRE(precision == GT_FLOAT || precision == GT_DOUBLE, GT_ERR_INVALID_ARGUMENT,
+                   "precision must be GT_FLOAT or GT_DOUBLE");
+        if (n == 0) return;
+        GT_REQUIRE(y && x, GT_ERR_INVALID_ARGUMENT, "null array");
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, size_t(1) << 20);
+        if (precision == GT_FLOAT)
+            hipLaunchKernelGGL(gt::add_kernel<float>, dim3(blocks), dim3(256), 0, S(s), (float*)y, (const float*)x,
+                               (long long)n);
+        else
+            hipLaunchKernelGGL(gt::add_kernel<double>, dim3(blocks), dim3(256), 0, S(s), (double*)y,
+                               (const double*)x, (long long)n);
+        GT_HIP(hipGetLastError());
+    });
+}
 gt_status gt_stream_synchronize(gt_stream_t s) { return guarded([&] { GT_HIP(hipStreamSynchronize(S(s))); }); }
 gt_status gt_device_synchronize(void) { return guarded([&] { GT_HIP(hipDeviceSynchronize()); }); }
 

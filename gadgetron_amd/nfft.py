@@ -262,3 +262,75 @@ class NFFTPlan:
 
     def synchronize(self, stream=None):
         check(lib().gt_stream_synchronize(_stream(stream)))
+
+
+class NFFTOperator:
+    """NFFTOperator<cuNDArray, REAL, D> (toolboxes/nfft/NFFTOperator.h:10-37,
+    NFFTOperator.hpp:13-98): a linear operator over one plan and an optional
+    dcw.  mult_M = compute(FORWARDS_C2NC), mult_MH = compute(BACKWARDS_NC2C),
+    mult_MH_M = the plan's mult_MH_M, each with the operator's dcw;
+    accumulate computes into a temporary and adds it to out (gt_array_add),
+    as the reference does with operator+=."""
+
+    def __init__(self):
+        self.plan = None
+        self.dcw = None
+        self.domain_dimensions = []
+        self.codomain_dimensions = []
+
+    def setup(self, matrix_size, matrix_size_os, W, precision="float"):
+        """NFFTOperator.hpp:84-89: make_plan(N, N_os, W)."""
+        self.plan = NFFTPlan(matrix_size, matrix_size_os, W, precision=precision)
+
+    def preprocess(self, trajectory, stream=None):
+        """NFFTOperator.hpp:91-95: NFFT_prep_mode.ALL."""
+        self._need_plan("preprocess")
+        self.plan.preprocess(trajectory, NFFT_prep_mode.ALL, stream)
+
+    def set_dcw(self, dcw):
+        self.dcw = dcw
+
+    def get_dcw(self):
+        return self.dcw
+
+    def get_plan(self):
+        return self.plan
+
+    def set_domain_dimensions(self, dims):
+        self.domain_dimensions = [int(d) for d in dims]
+
+    def set_codomain_dimensions(self, dims):
+        self.codomain_dimensions = [int(d) for d in dims]
+
+    def mult_M(self, inp, out, accumulate=False, stream=None):
+        self._apply("mult_M", inp, out, accumulate, stream,
+                    lambda dst: self.plan.compute(inp, dst, self.dcw, NFFT_comp_mode.FORWARDS_C2NC, stream))
+
+    def mult_MH(self, inp, out, accumulate=False, stream=None):
+        self._apply("mult_MH", inp, out, accumulate, stream,
+                    lambda dst: self.plan.compute(inp, dst, self.dcw, NFFT_comp_mode.BACKWARDS_NC2C, stream))
+
+    def mult_MH_M(self, inp, out, accumulate=False, stream=None):
+        """NFFTOperator.hpp:59-80: refuses to run before the codomain is set."""
+        if inp is not None and out is not None and not self.codomain_dimensions:
+            raise GTError(1, "NFFTOperator::mult_MH_M : operator codomain dimensions not set")
+        self._apply("mult_MH_M", inp, out, accumulate, stream,
+                    lambda dst: self.plan.mult_MH_M(inp, dst, self.dcw, stream))
+
+    def _need_plan(self, who):
+        if self.plan is None:
+            raise GTError(1, f"NFFTOperator::{who} : setup has not been called")
+
+    def _apply(self, who, inp, out, accumulate, stream, run):
+        if inp is None or out is None:
+            raise GTError(1, f"NFFTOperator::{who} : 0x0 input/output not accepted")
+        self._need_plan(who)
+        if not accumulate:
+            run(out)
+            return
+        dbl = self.plan.precision == "double"
+        tmp = DeviceArray(out.shape, np.complex128 if dbl else np.complex64)
+        run(tmp)
+        check(lib().gt_array_add(_ptr(out), _ptr(tmp), 2 * _numel(tmp), 1 if dbl else 0, _stream(stream)))
+        check(lib().gt_stream_synchronize(_stream(stream)))  # before tmp is freed
+        tmp.free()

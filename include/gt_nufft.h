@@ -160,6 +160,11 @@ gt_status gt_memcpy_htod(void* dst, const void* src, size_t bytes, gt_stream_t s
 gt_status gt_memcpy_dtoh(void* dst, const void* src, size_t bytes, gt_stream_t stream);
 gt_status gt_memcpy_dtod(void* dst, const void* src, size_t bytes, gt_stream_t stream);
 gt_status gt_memset(void* ptr, int value, size_t bytes, gt_stream_t stream);
+/* y += x over n real components (GT_FLOAT: float, GT_DOUBLE: double; complex
+ * arrays pass twice their element count): cuNDArray operator+=
+ * (core/gpu/cuNDArray_operators.cu:85-88), which NFFTOperator's accumulating
+ * applications use (NFFTOperator.hpp:22-33). */
+gt_status gt_array_add(void* y, const void* x, size_t n, int precision, gt_stream_t stream);
 gt_status gt_stream_synchronize(gt_stream_t stream);
 gt_status gt_device_synchronize(void);
 

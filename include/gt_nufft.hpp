@@ -22,6 +22,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "gt_nufft.h"
@@ -211,6 +212,85 @@ struct NFFT {
         ConvolutionType conv = ConvolutionType::STANDARD) {
         return std::make_shared<cuNFFT_plan<REAL, D>>(matrix_size, matrix_size_os, W, conv);
     }
+};
+
+// NFFTOperator<cuNDArray, REAL, D> (toolboxes/nfft/NFFTOperator.h:10-37,
+// NFFTOperator.hpp:13-98): a linear operator over one plan and an optional
+// dcw.  mult_M = compute(FORWARDS_C2NC), mult_MH = compute(BACKWARDS_NC2C),
+// mult_MH_M = the plan's mult_MH_M, each with the operator's dcw; accumulate
+// computes into a temporary and adds it to out (gt_array_add), as the
+// reference does with operator+=.
+template <class REAL, unsigned int D>
+class NFFTOperator {
+public:
+    using vec = std::array<size_t, D>;
+    using plan_type = cuNFFT_plan<REAL, D>;
+
+    virtual ~NFFTOperator() = default;
+
+    // setup (NFFTOperator.hpp:84-89): make_plan(N, N_os, W)
+    virtual void setup(const vec& matrix_size, const vec& matrix_size_os, REAL W) {
+        plan_ = NFFT<cuNDArray, REAL, D>::make_plan(matrix_size, matrix_size_os, W);
+    }
+    // preprocess (NFFTOperator.hpp:91-95): NFFT_prep_mode::ALL
+    template <class TRAJ>
+    void preprocess(const TRAJ& trajectory) {
+        need_plan("preprocess");
+        plan_->preprocess(trajectory, NFFT_prep_mode::ALL);
+    }
+    virtual void set_dcw(std::shared_ptr<cuNDArray<REAL>> dcw) { dcw_ = std::move(dcw); }
+    std::shared_ptr<cuNDArray<REAL>> get_dcw() { return dcw_; }
+    std::shared_ptr<plan_type> get_plan() { return plan_; }
+
+    // linearOperator's domain / codomain (operators/linearOperator.h)
+    void set_domain_dimensions(std::vector<size_t> d) { domain_ = std::move(d); }
+    void set_codomain_dimensions(std::vector<size_t> d) { codomain_ = std::move(d); }
+    std::vector<size_t> get_domain_dimensions() const { return domain_; }
+    std::vector<size_t> get_codomain_dimensions() const { return codomain_; }
+
+    template <class IN, class OUT>
+    void mult_M(IN* in, OUT* out, bool accumulate = false) {
+        apply("mult_M", in, out, accumulate, [&](auto& dst) {
+            plan_->compute(*in, dst, dcw_.get(), NFFT_comp_mode::FORWARDS_C2NC);
+        });
+    }
+    template <class IN, class OUT>
+    void mult_MH(IN* in, OUT* out, bool accumulate = false) {
+        apply("mult_MH", in, out, accumulate, [&](auto& dst) {
+            plan_->compute(*in, dst, dcw_.get(), NFFT_comp_mode::BACKWARDS_NC2C);
+        });
+    }
+    // NFFTOperator.hpp:59-80: refuses to run before the codomain is set
+    template <class IN, class OUT>
+    void mult_MH_M(IN* in, OUT* out, bool accumulate = false) {
+        if (in && out && codomain_.empty())
+            throw std::runtime_error("NFFTOperator::mult_MH_M : operator codomain dimensions not set");
+        apply("mult_MH_M", in, out, accumulate, [&](auto& dst) { plan_->mult_MH_M(*in, dst, dcw_.get()); });
+    }
+
+protected:
+    void need_plan(const char* who) const {
+        if (!plan_) throw std::runtime_error(std::string("NFFTOperator::") + who + " : setup has not been called");
+    }
+    template <class IN, class OUT, class F>
+    void apply(const char* who, IN* in, OUT* out, bool accumulate, F&& run) {
+        if (!in || !out)
+            throw std::runtime_error(std::string("NFFTOperator::") + who + " : 0x0 input/output not accepted");
+        need_plan(who);
+        if (!accumulate) {
+            run(*out);
+            return;
+        }
+        cuNDArray<std::complex<REAL>> tmp(detail::dims_of(*out));
+        run(tmp);
+        check(gt_array_add(out->get_data_ptr(), tmp.get_data_ptr(), 2 * tmp.get_number_of_elements(),
+                           std::is_same<REAL, double>::value ? GT_DOUBLE : GT_FLOAT, nullptr));
+        check(gt_stream_synchronize(nullptr));  // tmp is freed on return
+    }
+
+    std::shared_ptr<plan_type> plan_;
+    std::shared_ptr<cuNDArray<REAL>> dcw_;
+    std::vector<size_t> domain_, codomain_;
 };
 
 }  // namespace gt_amd

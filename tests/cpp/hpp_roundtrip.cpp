@@ -11,9 +11,15 @@
 //   writes <dir>/adjoint.bin   BACKWARDS_NC2C of samples, N*N*F*C complex64
 //          <dir>/forward.bin   FORWARDS_C2NC of adjoint.bin, M*F*C complex64
 //          <dir>/mhm.bin       mult_MH_M of adjoint.bin, N*N*F*C complex64
+//          <dir>/op_mh2.bin    NFFTOperator: mult_MH of samples, then again with
+//                              accumulate (twice adjoint.bin)
+//          <dir>/op_m.bin      NFFTOperator: mult_M of adjoint.bin (= forward.bin)
+//          <dir>/op_mhm2.bin   NFFTOperator: mult_MH_M of adjoint.bin, then again
+//                              with accumulate (twice mhm.bin)
 // and checks that invalid calls throw std::runtime_error.  Exit status 0 on success.
 #include <complex>
 #include <cstdio>
+#include <memory>
 #include <fstream>
 #include <iostream>
 #include <stdexcept>
@@ -106,8 +112,47 @@ int main(int argc, char** argv) {
         mhm.download(h_mhm.data());
         write_file(dir + "/mhm.bin", h_mhm);
 
-        // the reference's validation paths throw std::runtime_error
+        // NFFTOperator<cuNDArray, float, 2> over the same trajectory and weights
+        NFFTOperator<float, 2> op;
+        op.setup({N, N}, {N * 3 / 2, N * 3 / 2}, 5.5f);
+        op.preprocess(tv);
+        auto op_dcw = std::make_shared<cuNDArray<float>>(std::vector<size_t>{M, F});
+        op_dcw->upload(read_file<float>(dir + "/dcw.bin", M * F).data());
+        op.set_dcw(op_dcw);
+        auto save = [&](const std::string& name, const cuNDArray<cplx>& a) {
+            std::vector<cplx> h(a.get_number_of_elements());
+            a.download(h.data());
+            write_file(dir + "/" + name, h);
+        };
+        cuNDArray<cplx> op_img({N, N, F, C});
+        op.mult_MH(&samples, &op_img);
+        op.mult_MH(&samples, &op_img, true);
+        save("op_mh2.bin", op_img);
+        cuNDArray<cplx> op_fwd({M, F, C});
+        op.mult_M(&image, &op_fwd);
+        save("op_m.bin", op_fwd);
+        cuNDArray<cplx> op_mhm({N, N, F, C});
         bool threw = false;
+        try {
+            op.mult_MH_M(&image, &op_mhm);  // codomain not set yet
+        } catch (const std::runtime_error&) {
+            threw = true;
+        }
+        if (!threw) throw std::runtime_error("NFFTOperator::mult_MH_M without a codomain did not throw");
+        op.set_codomain_dimensions({M, F, C});
+        op.mult_MH_M(&image, &op_mhm);
+        op.mult_MH_M(&image, &op_mhm, true);
+        save("op_mhm2.bin", op_mhm);
+        threw = false;
+        try {
+            op.mult_M(static_cast<cuNDArray<cplx>*>(nullptr), &op_fwd);
+        } catch (const std::runtime_error&) {
+            threw = true;
+        }
+        if (!threw) throw std::runtime_error("NFFTOperator with a null input did not throw");
+
+        // the reference's validation paths throw std::runtime_error
+        threw = false;
         try {
             cuNDArray<cplx> wrong({N + 1, N, F, C});
             plan->compute(samples, wrong, &dcw, NFFT_comp_mode::BACKWARDS_NC2C);

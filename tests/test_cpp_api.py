@@ -61,6 +61,13 @@ def test_cpp_front_end_round_trip(gpu, tmp_path):
     assert rel_l2(img, ref_img) <= 1e-5
     assert rel_l2(fwd, ref_fwd) <= 1e-5
     assert rel_l2(mhm, ref_mhm) <= 1e-5
+    # NFFTOperator<float, 2>: mult_MH twice with accumulate, mult_M, mult_MH_M twice
+    op_mh2 = np.fromfile(tmp_path / "op_mh2.bin", np.complex64).reshape(C, 1, N, N)
+    op_m = np.fromfile(tmp_path / "op_m.bin", np.complex64).reshape(C, 1, M)
+    op_mhm2 = np.fromfile(tmp_path / "op_mhm2.bin", np.complex64).reshape(C, 1, N, N)
+    assert rel_l2(op_mh2, 2 * ref_img) <= 1e-5
+    assert rel_l2(op_m, ref_fwd) <= 1e-5
+    assert rel_l2(op_mhm2, 2 * ref_mhm) <= 1e-5
 
 
 PMRI_BIN = os.path.join(CPP, "pmri_roundtrip")
