@@ -504,4 +504,256 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// nc2c_mfma2d_kernel: the 2D NC2C of 32 coils per wave on the matrix cores.
+//
+// A tile is a product of two matrices: grid[q][cell] = sum over entries k of
+// s[q][k] * w[k][cell], with q = (coil, re/im) and w[k][cell] = wr_k[y] * wc_k[x].
+// Each chunk of 16 table entries is one K = 16 step of
+// v_mfma_f32_32x32x16_f16 on four 32 x 32 blocks (re / im rows x upper /
+// lower half tile).  FP32 operands are split into fp16 pairs, x = hi + lo with
+// hi = fp16(x) and lo = fp16(x - hi), and each block takes the three products
+// hi*hi + hi*lo + lo*hi: every fp16 x fp16 product is exact in the FP32
+// accumulator, and the dropped lo*lo term and the two roundings leave a
+// relative error of about 2^-22 per product.  fp16 has a narrow exponent
+// range, so the weights are normalised by a host power of two (wscale: the
+// largest product of the two 1-D kernels becomes <= 1) and the samples by a
+// power of two R kept per wave: R is set from the first chunk with nonzero
+// samples to put them below 2^12, and lowered (the accumulators rescaled
+// exactly) when a chunk would exceed 2^15.  Results are multiplied by
+// 1 / (R * wscale) on the way out.  Entry order, chunk order and the
+// hardware's product order are fixed: the result is bitwise repeatable.
+//
+// Operand lane layouts (32x32x16 f16): A holds row l % 32 at k = 8 (l / 32)
+// + i; B holds column l % 32 at the same k; accumulator register r of lane l
+// is row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
+// ---------------------------------------------------------------------------
+typedef _Float16 mf_half8 __attribute__((ext_vector_type(8)));
+typedef float mf_float8 __attribute__((ext_vector_type(8)));
+typedef float mf_float16 __attribute__((ext_vector_type(16)));
+
+// CW: coils per wave (32: two 32-row blocks, real and imaginary parts; 16: one
+// block, rows = (re / im, coil)).
+template <int CW>
+struct MfLds {
+    static constexpr int WS = kNK + 4;         // floats per row of a transposed table chunk
+    static constexpr int XS = kNK + 4;         // floats per sample row (re or im of one coil)
+    static constexpr int Tab = 2 * 16 * WS;    // two chunks: rows 0-7 column weights, 8-15 row weights
+    static constexpr int Heads = 2 * 2 * kNK;
+    static constexpr int Loop = Tab + Heads + 2 * CW * XS;
+    static constexpr int Out = kRT * kRT / 2 * kSubP;
+    static constexpr int Floats = ((Out > Loop ? Out : Loop) + 3) & ~3;
+};
+
+__device__ __forceinline__ void mf_split(const mf_float8& x, mf_half8& hi, mf_half8& lo) {
+    hi = __builtin_convertvector(x, mf_half8);
+    lo = __builtin_convertvector(x - __builtin_convertvector(hi, mf_float8), mf_half8);
+}
+
+__device__ __forceinline__ float mf_wave_max(float v) {
+    // v >= 0: the bit patterns order as the values do
+    uint32_t b = __float_as_uint(v);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) b = max(b, (uint32_t)__shfl_xor((int)b, o));
+    return __uint_as_float(b);
+}
+
+__device__ __forceinline__ float mf_absmax(const mf_float8& x) {
+    return fmaxf(fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3]))),
+                 fmaxf(fmaxf(fabsf(x[4]), fabsf(x[5])), fmaxf(fabsf(x[6]), fabsf(x[7]))));
+}
+
+template <int DM, int CW>
+__global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
+    const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
+    const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
+    int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
+    int accumulate, float wscale) {
+    static_assert(CW == 32 || CW == 16, "coils per wave");
+    constexpr int LC = CW / 4;       // coils per lane in a chunk's sample loads
+    constexpr int MB = 2 * CW / 32;  // 32-row blocks of (coil, re / im)
+    using Lds = MfLds<CW>;
+    constexpr int WS = Lds::WS, XS = Lds::XS;
+    __shared__ float4 smem[Lds::Floats / 4];
+    float* xp = reinterpret_cast<float*>(smem);                 // write-out transpose
+    float* wT = reinterpret_cast<float*>(smem);                 // [2][16][WS]
+    uint32_t* sh = reinterpret_cast<uint32_t*>(wT + Lds::Tab);  // [2][2 kNK] heads
+    float* xs = reinterpret_cast<float*>(sh + Lds::Heads);      // [re / im][coil][XS] per entry
+    const int lane = threadIdx.x;
+    const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
+    const int c0 = blockIdx.y * CW;
+    const long long FM = (long long)F * M;
+    const uint32_t nch = (it.end - it.begin + kNK - 1) / kNK;
+    const int kk = lane & (kNK - 1), cq = lane >> 4;  // sample loads: (entry, coil quarter)
+    const int col = lane & 31, hk = lane >> 5;         // MFMA: row / column l % 32, k half
+    const float2* s2 = reinterpret_cast<const float2*>(samples);
+
+    mf_float16 acc[MB][2];
+#pragma unroll
+    for (int a = 0; a < MB; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[a][b][r] = 0.0f;
+    float R = 0.0f;  // sample scale (0: not set yet)
+
+    constexpr int kDV = DM == 2 ? LC : 1;
+    float4 wreg;
+    uint32_t hreg;
+    float2 sv[LC];
+    float dv[kDV];
+#pragma unroll
+    for (int u = 0; u < LC; u++) sv[u] = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < kDV; u++) dv[u] = 1.0f;
+
+    auto stage = [&](int b) {
+        // lane = (entry e, part p): part 0 / 1 column weights 0-3 / 4-7, 2 / 3 row weights
+        const int e = lane >> 2, pr = lane & 3;
+        float* dst = wT + b * 16 * WS + 4 * pr * WS + e;
+        dst[0] = wreg.x;
+        dst[WS] = wreg.y;
+        dst[2 * WS] = wreg.z;
+        dst[3 * WS] = wreg.w;
+        if (lane < 2 * kNK) sh[b * 2 * kNK + lane] = hreg;
+    };
+    auto issue = [&](int b) {
+        const uint32_t j = sh[b * 2 * kNK + 2 * kk];
+#pragma unroll
+        for (int u = 0; u < LC; u++) {
+            int c = c0 + cq * LC + u;
+            c = c < C ? c : c0;  // coils past the last one repeat a valid coil, never stored
+            sv[u] = s2[(long long)c * FM + j];
+            if constexpr (DM == 2) dv[u] = dcw[dcw_index(c, j, FM, dcw_len)];
+        }
+        if constexpr (DM == 1) dv[0] = dcw[j];
+    };
+    auto transpose = [&]() {
+#pragma unroll
+        for (int u = 0; u < LC; u++) {
+            float2 v = sv[u];
+            if constexpr (DM != 0) {
+                const float d = dv[DM == 2 ? u : 0];
+                v.x *= d;
+                v.y *= d;
+            }
+            const int cl = cq * LC + u;
+            xs[cl * XS + kk] = v.x;
+            xs[(CW + cl) * XS + kk] = v.y;
+        }
+    };
+    auto grid_chunk = [&](int b) {
+        // A: rows a * 32 + col (re / im, coil), entries 8 hk .. 8 hk + 7
+        mf_float8 x[MB];
+        const float Rt = R == 0.0f ? 1.0f : R;
+        float m = 0.0f;
+#pragma unroll
+        for (int a = 0; a < MB; a++) {
+            const float4* xa = reinterpret_cast<const float4*>(xs + (a * 32 + col) * XS + 8 * hk);
+            const float4 a0 = xa[0], a1 = xa[1];
+            x[a] = mf_float8{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w} * Rt;
+            m = fmaxf(m, mf_absmax(x[a]));
+        }
+        if (__builtin_amdgcn_ballot_w64(m > 32768.0f || (R == 0.0f && m > 0.0f))) {
+            const float mx = mf_wave_max(m);  // of x = samples * Rt
+            int e;
+            frexpf(mx, &e);                   // mx < 2^e
+            e = min(max(e, -100), 140);       // keep R and 1 / R finite (denormal or huge samples)
+            const float f = ldexpf(1.0f, 12 - e);  // x * f < 2^12; R * f is the new scale
+            if (R != 0.0f) {
+#pragma unroll
+                for (int a = 0; a < MB; a++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) acc[a][h] *= f;  // powers of two: exact
+            }
+#pragma unroll
+            for (int a = 0; a < MB; a++) x[a] *= f;
+            R = Rt * f;
+        }
+        mf_half8 xh[MB], xl[MB];
+#pragma unroll
+        for (int a = 0; a < MB; a++) mf_split(x[a], xh[a], xl[a]);
+        // B: column weight of this lane's cell x, row weights of its y in both half tiles
+        const float* wb = wT + b * 16 * WS + 8 * hk;
+        const int cx = col & 7, cy = col >> 3;
+        const float4* pc = reinterpret_cast<const float4*>(wb + cx * WS);
+        const float4* p0 = reinterpret_cast<const float4*>(wb + (8 + cy) * WS);
+        const float4* p1 = reinterpret_cast<const float4*>(wb + (12 + cy) * WS);
+        const float4 c_0 = pc[0], c_1 = pc[1], r00 = p0[0], r01 = p0[1], r10 = p1[0], r11 = p1[1];
+        const mf_float8 wc = mf_float8{c_0.x, c_0.y, c_0.z, c_0.w, c_1.x, c_1.y, c_1.z, c_1.w} * wscale;
+        const mf_float8 w0 = wc * mf_float8{r00.x, r00.y, r00.z, r00.w, r01.x, r01.y, r01.z, r01.w};
+        const mf_float8 w1 = wc * mf_float8{r10.x, r10.y, r10.z, r10.w, r11.x, r11.y, r11.z, r11.w};
+        mf_half8 w0h, w0l, w1h, w1l;
+        mf_split(w0, w0h, w0l);
+        mf_split(w1, w1h, w1l);
+#pragma unroll
+        for (int a = 0; a < MB; a++) {
+            acc[a][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl[a], w0h, acc[a][0], 0, 0, 0);
+            acc[a][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl[a], w1h, acc[a][1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < MB; a++) {
+            acc[a][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh[a], w0l, acc[a][0], 0, 0, 0);
+            acc[a][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh[a], w1l, acc[a][1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < MB; a++) {
+            acc[a][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh[a], w0h, acc[a][0], 0, 0, 0);
+            acc[a][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh[a], w1h, acc[a][1], 0, 0, 0);
+        }
+    };
+
+    if (nch > 0) {
+        nc2c_chunk_load(wt, hd, it.begin, it.end, lane, wreg, hreg);
+        stage(0);
+        nc2c_chunk_load(wt, hd, it.begin + kNK, it.end, lane, wreg, hreg);
+        issue(0);
+        for (uint32_t c = 0; c < nch; c++) {
+            const int b = (int)(c & 1u);
+            transpose();
+            stage(b ^ 1);
+            nc2c_chunk_load(wt, hd, it.begin + (c + 2) * kNK, it.end, lane, wreg, hreg);
+            issue(b ^ 1);
+            grid_chunk(b);
+        }
+    }
+
+    // write-out through LDS, half a tile (4 rows) at a time, as nc2c_reg2d_kernel
+    const float outs = R > 0.0f ? 1.0f / (R * wscale) : 0.0f;  // powers of two: exact
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    const int t = (int)(it.key % (uint32_t)g.ntiles_total);
+    const int ox = (t % g.ntiles[0]) * kRT, oy = (t / g.ntiles[0]) * kRT;
+    const int lr = lane >> 4, li = lane & 15;
+    const int lx = li >> 1, comp = li & 1;
+    const int gx = ox + lx;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < MB; a++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int row = a * 32 + 8 * (r >> 2) + 4 * hk + (r & 3);  // (re / im, coil)
+                xp[col * kSubP + 2 * (row % CW) + row / CW] = acc[a][half][r] * outs;
+            }
+        __syncthreads();
+        for (int r = lr; r < CW * (kRT / 2); r += 4) {
+            const int cl = r >> 2, ly = half * (kRT / 2) + (r & 3);
+            const int cc = c0 + cl;
+            const int gy = oy + ly;
+            if (cc >= C || gx >= g.nos[0] || gy >= g.nos[1]) continue;
+            float v = xp[(((r & 3) * kRT) + lx) * kSubP + 2 * cl + comp];
+            if (it.partial >= 0) {
+                partials[(((long long)it.partial * C + cc) * (kRT * kRT) + ly * kRT + lx) * 2 + comp] = v;
+                continue;
+            }
+            if (sign && ((gx + gy) & 1)) v = -v;
+            const long long o = 2 * ((long long)gx + L.sy * gy + L.sb * ((long long)cc * F + f)) + comp;
+            if (accumulate) v += grid[o];
+            grid[o] = v;
+        }
+    }
+}
+
 }  // namespace gt

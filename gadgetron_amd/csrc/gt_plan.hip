@@ -612,6 +612,34 @@ struct LaunchTablesC2NC {
     }
 };
 
+// The 2D NC2C of 32 coils per wave runs on the matrix cores unless
+// GT_NUFFT_MFMA=0 (the FP32 register-tile kernel, kept for comparison).
+bool nc2c_on_matrix_cores() {
+    static const bool on = [] {
+        const char* e = std::getenv("GT_NUFFT_MFMA");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// Coils per wave of the matrix-core NC2C (GT_NUFFT_MFMA_CW=16 or 32).
+int mfma_coils_per_wave() {
+    static const int cw = [] {
+        const char* e = std::getenv("GT_NUFFT_MFMA_CW");
+        return e && std::atoi(e) == 32 ? 32 : 16;
+    }();
+    return cw;
+}
+
+// Power of two that brings the largest product of the two 1-D kernel weights
+// (at u = 0) to at most 1: the weight normaliser of nc2c_mfma2d_kernel.
+float mfma_weight_scale(const gt_nufft_plan* p) {
+    const gt::GridGeom& g = p->geom;
+    const double peak = double(gt::kb1(0.0f, g.nos_f[0], g.one_over_W, g.beta[0])) *
+                        double(gt::kb1(0.0f, g.nos_f[1], g.one_over_W, g.beta[1]));
+    return std::ldexp(1.0f, -int(std::ceil(std::log2(peak))));
+}
+
 // Coil groups (NC2C) / sample slots (C2NC) of the 2D gridding kernels: a wave
 // covers 32 / G coils, so fewer coils leave no lanes idle.
 int coil_groups(int C) { return C <= 8 ? 4 : (C <= 16 ? 2 : 1); }
@@ -630,6 +658,20 @@ struct LaunchFastNC2C {
         const int cw = gt::kCoilsPerWave / G;
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
         const long long FM = (long long)(p->M * p->F);
+        if (G == 1 && nc2c_on_matrix_cores()) {
+            // the tile product on the matrix cores, CW coils per wave
+            const int cwm = mfma_coils_per_wave();
+            const unsigned mchunks = (unsigned)((C + cwm - 1) / cwm);
+            auto km = cwm == 16 ? (!dcw ? gt::nc2c_mfma2d_kernel<0, 16>
+                                        : (dcw_len == FM ? gt::nc2c_mfma2d_kernel<1, 16> : gt::nc2c_mfma2d_kernel<2, 16>))
+                                : (!dcw ? gt::nc2c_mfma2d_kernel<0, 32>
+                                        : (dcw_len == FM ? gt::nc2c_mfma2d_kernel<1, 32> : gt::nc2c_mfma2d_kernel<2, 32>));
+            hipLaunchKernelGGL(km, dim3((unsigned)p->bn.n_items, mchunks), dim3(64), 0, s,
+                               samples, grid, partials, p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(),
+                               p->bn.items.as<gt::WorkItem>(), int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw,
+                               dcw_len, accumulate, mfma_weight_scale(p));
+            return;
+        }
         auto k = G == 4 ? pick<4>(dcw, dcw_len, FM) : (G == 2 ? pick<2>(dcw, dcw_len, FM) : pick<1>(dcw, dcw_len, FM));
         hipLaunchKernelGGL(k, dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s,
                            samples, grid, partials, p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(),
@@ -1008,7 +1050,8 @@ size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& count
         return p->max_items_req > 0 ? std::min(cap, size_t(p->max_items_req)) : cap;
     }
     if (p->max_items_req > 0) return size_t(p->max_items_req);
-    if (p->fast2d) return 256;
+    // the matrix-core NC2C spends less per entry: longer items, fewer partial tiles
+    if (p->fast2d) return nc2c_on_matrix_cores() ? 512 : 256;
     if (p->fast3d) return 1024;
     const double avg = double(npairs) / double(std::max<size_t>(counts.size(), 1));
     return size_t(std::max(256.0, 4.0 * avg));
