@@ -119,6 +119,49 @@ __global__ void c2nc_table2d_kernel(const float* __restrict__ p, const uint32_t*
     write_entry<KW>(tab + e * kTabDw, j, pack_entry(bx, by, mx, my), wx, wy, mx, my);
 }
 
+// Weight rows of the matrix-core C2NC, 32 floats per entry: the sample's x
+// weights at the 16 staged columns, then its y weights at the 16 staged rows
+// (staged coordinates start at tile origin - floor(r); zero outside the
+// window), each axis scaled by a power of two (sx, sy) so no weight exceeds 1.
+template <int KW>
+__global__ void c2nc_wtable2d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list, long long n,
+                                     GridGeom g, float sx, float sy, float* __restrict__ wtab) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    static_assert(8 + KW - 1 <= 16, "a staged row holds 16 cells");
+    const uint32_t j = list[e];
+    const int fr = (int)floorf(g.radius);
+    float wx[KW], wy[KW];
+    int lox, loy;
+    float px, py;
+    const int tx = home_tile_1d(p[(long long)j * 2 + 0], g, 0, px);
+    const int ty = home_tile_1d(p[(long long)j * 2 + 1], g, 1, py);
+    window_raw<KW>(px, g, 0, wx, lox);
+    window_raw<KW>(py, g, 1, wy, loy);
+    const int bx = min(max(lox - (tx * g.tile[0] - fr), 0), g.tile[0]);
+    const int by = min(max(loy - (ty * g.tile[1] - fr), 0), g.tile[1]);
+    float rx[16], ry[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        rx[c] = 0.0f;
+        ry[c] = 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < KW; i++) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            if (c == bx + i) rx[c] = wx[i] * sx;
+            if (c == by + i) ry[c] = wy[i] * sy;
+        }
+    }
+    float4* t = reinterpret_cast<float4*>(wtab + e * 32);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        t[q] = make_float4(rx[4 * q], rx[4 * q + 1], rx[4 * q + 2], rx[4 * q + 3]);
+        t[4 + q] = make_float4(ry[4 * q], ry[4 * q + 1], ry[4 * q + 2], ry[4 * q + 3]);
+    }
+}
+
 // NC2C table, per (8 x 8 tile, sample) pair e, in two arrays:
 //   wt[16 e + c]     weight of tile column c (c = 0..7), then of tile row c - 8
 //                    (zero where the window does not reach)

@@ -756,4 +756,167 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// c2nc_mfma2d_kernel: the 2D C2NC of 32 coils on the matrix cores.
+//
+// The samples of a home tile are a product: s[q][n] = sum over staged cells k
+// of grid[q][k] * w[k][n], q = (coil, re/im), n = sample, w[k][n] = wy_n[y] *
+// wx_n[x].  A workgroup of four waves stages the tile + halo (E x E cells, E =
+// 8 + KW - 1, rows padded to 16 cells) once, scaled by a power of two R and
+// split into fp16 hi / lo planes in LDS; each wave then takes 32 samples and
+// walks the E staged rows as K = 16 steps of v_mfma_f32_32x32x16_f16 on two
+// 32-row blocks (coils 0-15, 16-31, rows 2 coil + re/im), with the three
+// products hi*hi + hi*lo + lo*hi as in nc2c_mfma2d_kernel.  The B operand is
+// the sample's weight row product, from a table that holds each sample's
+// weights at staged coordinates (c2nc_wtable2d_kernel).  Out: s * oscale / R
+// (oscale undoes the table's power-of-two weight scales), then dcw.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int c2nc_mfma_row(int KW) { return (kRT + KW - 1) * 16 + 8; }  // halves per A row
+__host__ __device__ constexpr int c2nc_mfma_lds_bytes(int KW, int CW) {
+    return 2 * (2 * CW) * c2nc_mfma_row(KW) * 2 + 64;
+}
+
+// CW: coils per workgroup (32 or 16: two or one 32-row blocks).
+template <int KW, int CW>
+__global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
+    const float* __restrict__ grid, float* __restrict__ out, const float* __restrict__ wtab,
+    const uint32_t* __restrict__ list, const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g,
+    Layout L, int sign, const float* __restrict__ dcw, long long dcw_len, int accumulate, float oscale) {
+    constexpr int E = kRT + KW - 1;  // staged cells per dimension
+    constexpr int AS = c2nc_mfma_row(KW);
+    constexpr int MB = 2 * CW / 32;  // 32-row blocks
+    static_assert(E <= 16 && (CW == 32 || CW == 16), "staged rows of 16 cells, 16 or 32 coils");
+    extern __shared__ float4 c2m_smem[];
+    _Float16* Ahi = reinterpret_cast<_Float16*>(c2m_smem);
+    _Float16* Alo = Ahi + 2 * CW * AS;
+    float* red = reinterpret_cast<float*>(Alo + 2 * CW * AS);  // four wave maxima
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
+    const int c0 = blockIdx.y * CW;
+    const int fr = (int)floorf(g.radius);
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    const int t = (int)(it.key % (uint32_t)g.ntiles_total);
+    const int sx = (t % g.ntiles[0]) * kRT - fr, sy0 = (t / g.ntiles[0]) * kRT - fr;
+    const uint32_t ne = it.end - it.begin;  // at most 4 x 32
+
+    // staging: thread = (cell column ex, re/im) x coil group; columns past E
+    // write zeros, so every staged row holds 16 defined cells
+    constexpr int kU = CW / 8;
+    const int q = tid & 31, cg = tid >> 5;
+    const int ex = q >> 1, comp = q & 1;
+    const bool live = ex < E;
+    int gx = sx + ex;
+    gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
+    const long long cstride = 2 * L.sb * (long long)F;  // next coil
+    const float* gcol = grid + 2 * (long long)(live ? gx : 0) + comp + 2 * L.sb * (long long)f;
+    float v[E][kU];
+    float mloc = 0.0f;
+#pragma unroll
+    for (int ey = 0; ey < E; ey++) {
+        int gy = sy0 + ey;
+        gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
+        const float* prow = gcol + 2 * L.sy * (long long)gy;
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int cl = cg + 8 * u;
+            v[ey][u] = (live && c0 + cl < C) ? prow[(long long)(c0 + cl) * cstride] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int ey = 0; ey < E; ey++) {
+        const int gy = sy0 + ey;  // parity of the wrapped row equals that of gy (even N_os)
+        const float sgn = (sign && ((gx + gy) & 1)) ? -1.0f : 1.0f;
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            v[ey][u] *= sgn;
+            mloc = fmaxf(mloc, fabsf(v[ey][u]));
+        }
+    }
+    const float wm = mf_wave_max(mloc);
+    if (lane == 0) red[wave] = wm;
+    __syncthreads();
+    const float mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    int e2;
+    frexpf(mx, &e2);               // mx < 2^e2
+    e2 = min(max(e2, -100), 140);  // keep R and 1 / R finite
+    const float R = mx > 0.0f ? ldexpf(1.0f, 12 - e2) : 1.0f;
+#pragma unroll
+    for (int ey = 0; ey < E; ey++)
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const float x = v[ey][u] * R;
+            const _Float16 h = (_Float16)x;
+            const _Float16 l = (_Float16)(x - (float)h);
+            const int idx = (2 * (cg + 8 * u) + comp) * AS + ey * 16 + ex;  // row 2 coil + re/im
+            Ahi[idx] = h;
+            Alo[idx] = l;
+        }
+    __syncthreads();
+
+    if ((uint32_t)(wave * 32) >= ne) return;  // wave-uniform; no barrier follows
+    const int col = lane & 31, hk = lane >> 5;
+    const uint32_t e = (uint32_t)(wave * 32 + col);
+    const bool on = e < ne;
+    const float4* wr = reinterpret_cast<const float4*>(wtab + (size_t)(it.begin + (on ? e : 0u)) * 32);
+    const float4 xa = wr[2 * hk], xb = wr[2 * hk + 1];
+    const float4 y0 = wr[4], y1 = wr[5], y2 = wr[6], y3 = wr[7];
+    const float z = on ? 1.0f : 0.0f;
+    const mf_float8 wx = mf_float8{xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w} * z;
+    const float wy[16] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w,
+                          y2.x, y2.y, y2.z, y2.w, y3.x, y3.y, y3.z, y3.w};
+    mf_float16 acc[MB];
+#pragma unroll
+    for (int a = 0; a < MB; a++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[a][r] = 0.0f;
+    const _Float16* ah = Ahi + col * AS + 8 * hk;
+    const _Float16* al = Alo + col * AS + 8 * hk;
+#pragma unroll
+    for (int sy = 0; sy < E; sy++) {
+        mf_half8 bh, bl;
+        mf_split(wx * wy[sy], bh, bl);
+        mf_half8 xh[MB], xl[MB];
+#pragma unroll
+        for (int a = 0; a < MB; a++) {
+            xh[a] = *reinterpret_cast<const mf_half8*>(ah + a * 32 * AS + sy * 16);
+            xl[a] = *reinterpret_cast<const mf_half8*>(al + a * 32 * AS + sy * 16);
+        }
+#pragma unroll
+        for (int a = 0; a < MB; a++) acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl[a], bh, acc[a], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < MB; a++) acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh[a], bl, acc[a], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < MB; a++) acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh[a], bh, acc[a], 0, 0, 0);
+    }
+    if (!on) return;
+    // accumulator register r of block a: row 32 a + 8 (r / 4) + 4 hk + r % 4 =
+    // 2 coil + re/im, column = this lane's sample: (r, r + 1) for even r are
+    // one coil's (re, im)
+    const long long FM = (long long)F * M;
+    const uint32_t j = list[it.begin + e];
+    const int dm = dcw_mode(dcw, dcw_len, FM);
+    float d = oscale / R;
+    if (dm == 1) d *= dcw[j];
+    float2* o2 = reinterpret_cast<float2*>(out);
+#pragma unroll
+    for (int a = 0; a < MB; a++)
+#pragma unroll
+        for (int r = 0; r < 16; r += 2) {
+            const int cl = (32 * a + 8 * (r >> 2) + 4 * hk + (r & 3)) >> 1;
+            const int cc = c0 + cl;
+            if (cc >= C) continue;
+            float dd = d;
+            if (dm == 2) dd *= dcw[dcw_index(cc, j, FM, dcw_len)];
+            float2 val = make_float2(acc[a][r] * dd, acc[a][r + 1] * dd);
+            float2* o = o2 + (long long)cc * FM + j;
+            if (accumulate) {
+                const float2 old = *o;
+                val.x += old.x;
+                val.y += old.y;
+            }
+            *o = val;
+        }
+}
+
 }  // namespace gt

@@ -119,6 +119,7 @@ struct Binning {
     DevBuf table;     // per-entry Kaiser-Bessel table (2D hot path)
     DevBuf heads;     // per-entry (sample index, row mask) of the NC2C table
     DevBuf dcw_e;     // dcw per entry (3D C2NC, per-sample dcw)
+    DevBuf wtab;      // per-entry weight rows of the matrix-core 2D C2NC
     DevBuf items;     // gt::WorkItem
     DevBuf red;       // gt::RedItem (split tiles)
     size_t n = 0;
@@ -126,6 +127,7 @@ struct Binning {
     void release() {
         list.release();
         dcw_e.release();
+        wtab.release();
         table.release();
         heads.release();
         items.release();
@@ -631,6 +633,14 @@ int mfma_coils_per_wave() {
     return cw;
 }
 
+// Power of two that brings the largest 1-D kernel weight along axis d (at
+// u = 0) to at most 1.
+float kb_axis_scale(const gt_nufft_plan* p, int d) {
+    const gt::GridGeom& g = p->geom;
+    const double peak = double(gt::kb1(0.0f, g.nos_f[d], g.one_over_W, g.beta[d]));
+    return std::ldexp(1.0f, -int(std::ceil(std::log2(peak))));
+}
+
 // Power of two that brings the largest product of the two 1-D kernel weights
 // (at u = 0) to at most 1: the weight normaliser of nc2c_mfma2d_kernel.
 float mfma_weight_scale(const gt_nufft_plan* p) {
@@ -643,6 +653,15 @@ float mfma_weight_scale(const gt_nufft_plan* p) {
 // Coil groups (NC2C) / sample slots (C2NC) of the 2D gridding kernels: a wave
 // covers 32 / G coils, so fewer coils leave no lanes idle.
 int coil_groups(int C) { return C <= 8 ? 4 : (C <= 16 ? 2 : 1); }
+
+template <int KW>
+struct LaunchWTableC2NC {
+    static void run(const float* pos, const uint32_t* list, long long n, const gt::GridGeom& g, float sx, float sy,
+                    float* wtab, hipStream_t s) {
+        hipLaunchKernelGGL((gt::c2nc_wtable2d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos,
+                           list, n, g, sx, sy, wtab);
+    }
+};
 
 template <int KW>
 struct LaunchFastNC2C {
@@ -693,6 +712,19 @@ struct LaunchFastC2NC {
         const int G = coil_groups(C);
         const int cw = gt::kCoilsPerWave / G;
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
+        if (G == 1 && b.wtab.ptr && nc2c_on_matrix_cores()) {
+            // 17 coils or more: the home tiles' products on the matrix cores,
+            // cwm coils per workgroup
+            const float oscale = 1.0f / (kb_axis_scale(p, 0) * kb_axis_scale(p, 1));  // powers of two
+            const int cwm = mfma_coils_per_wave();
+            const unsigned mchunks = (unsigned)((C + cwm - 1) / cwm);
+            auto km = cwm == 16 ? gt::c2nc_mfma2d_kernel<KW, 16> : gt::c2nc_mfma2d_kernel<KW, 32>;
+            hipLaunchKernelGGL(km, dim3((unsigned)b.n_items, mchunks), dim3(256),
+                               size_t(gt::c2nc_mfma_lds_bytes(KW, cwm)), s, grid, out, b.wtab.as<float>(),
+                               b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), int(p->M), int(p->F), C, b.geom, L,
+                               sign, dcw, dcw_len, accumulate, oscale);
+            return;
+        }
         const size_t lds = size_t(gt::c2nc_lds_bytes(KW, G));
         const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
         // a per-sample dcw is gathered per table entry inside the kernel
@@ -1101,6 +1133,16 @@ void build_c2nc(gt_nufft_plan* p, hipStream_t s) {
         b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
         dispatch_fast<LaunchTablesC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
                                         b.geom, b.table.as<uint32_t>(), s);
+        if (nc2c_on_matrix_cores()) {
+            // weight rows for the matrix-core C2NC (32 coils per wave)
+            b.wtab.reserve(std::max<size_t>(b.n, 1) * 32 * sizeof(float));
+            const float sx = kb_axis_scale(p, 0), sy = kb_axis_scale(p, 1);
+            if (b.n)
+                dispatch_fast<LaunchWTableC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
+                                                b.geom, sx, sy, b.wtab.as<float>(), s);
+        } else {
+            b.wtab.release();
+        }
     } else {
         b.table.reserve(std::max<size_t>(b.n, 1) * gt::k3CDw * 4);
         dispatch_fast<LaunchTablesC2NC3>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
