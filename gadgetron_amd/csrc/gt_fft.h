@@ -207,6 +207,25 @@ constexpr int kFftRowThreads = 64;  // threads per row transform
 constexpr int kFftRows = 4;         // rows per workgroup
 constexpr int kFftColW = 16;        // columns per workgroup (one 128-byte run of a row)
 
+// Columns per workgroup of the column FFT of length OY: 8 (a 64-byte half run,
+// half the LDS, so more workgroups per CU) where 32 threads per column divide
+// the length, else 16.
+template <int OY>
+__host__ __device__ constexpr int fft_col_width() { return OY % 32 == 0 ? 8 : kFftColW; }
+
+// Column block of launch index i: with 8-column blocks the two halves of a
+// 128-byte run are blocks i and i + 8, which run on the same XCD (workgroup i
+// goes to XCD i mod 8) and share its L2.  Returns false past the last block.
+template <int W>
+__device__ __forceinline__ bool fft_col_block(int i, int nb, int nxb, int& b, int& xb) {
+    int lin = i;
+    if (W == 8) lin = ((i >> 4) * 8 + (i & 7)) * 2 + ((i >> 3) & 1);
+    if (lin >= nb * nxb) return false;
+    b = lin / nxb;
+    xb = lin % nxb;
+    return true;
+}
+
 // Forwards: image row (b, iy) -> grid row (iy + offy) * nb + b, padded to Ox
 // at offx, deapodized, centring-signed and scaled, then transformed along x.
 template <int OX>
@@ -319,13 +338,15 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_crop_kernel
 template <int OY, bool BAND_IN, bool BAND_OUT>
 __global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid, int Ox, FftGeom fg,
                                                        const float2* __restrict__ tw, float s) {
-    constexpr int W = kFftColW;
+    constexpr int W = fft_col_width<OY>();
     constexpr int TPS = 256 / W;
     __shared__ float2 sbuf[OY * W];
     __shared__ float2 stw[OY];
-    load_twiddles<OY>(stw, tw);
     const int nxb = (Ox + W - 1) / W;
-    const int b = blockIdx.x / nxb, x0 = (blockIdx.x % nxb) * W;
+    int b, xb;
+    if (!fft_col_block<W>((int)blockIdx.x, fg.nb, nxb, b, xb)) return;  // block-uniform, before any barrier
+    load_twiddles<OY>(stw, tw);
+    const int x0 = xb * W;
     const int xl = threadIdx.x % W, yr = threadIdx.x / W;
     const int x = x0 + xl;
     const bool xin = x < Ox;

@@ -1445,16 +1445,20 @@ struct LaunchRowsCrop {
 template <int L>
 struct LaunchColsBandIn {
     static void run(float2* grid, int Ox, const gt::FftGeom& fg, const float2* tw, float dir, hipStream_t s) {
-        const unsigned blocks = unsigned(size_t(fg.nb) * ((Ox + gt::kFftColW - 1) / gt::kFftColW));
-        hipLaunchKernelGGL((gt::fft_cols_kernel<L, true, false>), dim3(blocks), dim3(256), 0, s, grid, Ox, fg, tw,
+        constexpr int W = gt::fft_col_width<L>();
+        size_t blocks = size_t(fg.nb) * size_t((Ox + W - 1) / W);
+        if (W == 8) blocks = (blocks + 15) / 16 * 16;  // whole groups of the XCD-paired order
+        hipLaunchKernelGGL((gt::fft_cols_kernel<L, true, false>), dim3(unsigned(blocks)), dim3(256), 0, s, grid, Ox, fg, tw,
                            dir);
     }
 };
 template <int L>
 struct LaunchColsBandOut {
     static void run(float2* grid, int Ox, const gt::FftGeom& fg, const float2* tw, float dir, hipStream_t s) {
-        const unsigned blocks = unsigned(size_t(fg.nb) * ((Ox + gt::kFftColW - 1) / gt::kFftColW));
-        hipLaunchKernelGGL((gt::fft_cols_kernel<L, false, true>), dim3(blocks), dim3(256), 0, s, grid, Ox, fg, tw,
+        constexpr int W = gt::fft_col_width<L>();
+        size_t blocks = size_t(fg.nb) * size_t((Ox + W - 1) / W);
+        if (W == 8) blocks = (blocks + 15) / 16 * 16;  // whole groups of the XCD-paired order
+        hipLaunchKernelGGL((gt::fft_cols_kernel<L, false, true>), dim3(unsigned(blocks)), dim3(256), 0, s, grid, Ox, fg, tw,
                            dir);
     }
 };
