@@ -87,3 +87,14 @@ def test_python_errors_are_runtime_errors():
 def test_device_count_without_gpu_is_graceful():
     n = ga.device_count()
     assert n >= 0
+
+
+def test_array_add_validates_without_touching_memory():
+    """gt_array_add (cuNDArray operator+=): an empty add is a no-op and an
+    unknown precision is refused before anything is launched."""
+    L = lib()
+    assert L.gt_array_add(None, None, 0, 0, None) == 0
+    assert L.gt_array_add(None, None, 0, 1, None) == 0
+    assert L.gt_array_add(None, None, 4, 7, None) == 1
+    assert L.gt_last_error()
+    assert L.gt_array_add(None, None, 4, 0, None) == 1  # null arrays with n > 0
