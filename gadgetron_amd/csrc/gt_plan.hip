@@ -677,7 +677,8 @@ struct LaunchFastNC2C {
         const int cw = gt::kCoilsPerWave / G;
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
         const long long FM = (long long)(p->M * p->F);
-        if (G == 1 && nc2c_on_matrix_cores()) {
+        // 9 coils or more fill the 16-coil matrix-core kernel's rows at least half way
+        if ((G == 1 || (G == 2 && mfma_coils_per_wave() == 16)) && nc2c_on_matrix_cores()) {
             // the tile product on the matrix cores, CW coils per wave
             const int cwm = mfma_coils_per_wave();
             const unsigned mchunks = (unsigned)((C + cwm - 1) / cwm);
@@ -714,7 +715,8 @@ struct LaunchFastC2NC {
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
         if (G == 1 && b.wtab.ptr && nc2c_on_matrix_cores()) {
             // 17 coils or more: the home tiles' products on the matrix cores,
-            // cwm coils per workgroup
+            // cwm coils per workgroup (9 to 16 coils measured faster on the
+            // FP32 gather: 20.6 against 22.7 us at 256^2, 16 coils)
             const float oscale = 1.0f / (kb_axis_scale(p, 0) * kb_axis_scale(p, 1));  // powers of two
             const int cwm = mfma_coils_per_wave();
             const unsigned mchunks = (unsigned)((C + cwm - 1) / cwm);
