@@ -800,27 +800,27 @@ __global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
     const int sx = (t % g.ntiles[0]) * kRT - fr, sy0 = (t / g.ntiles[0]) * kRT - fr;
     const uint32_t ne = it.end - it.begin;  // at most 4 x 32
 
-    // staging: thread = (cell column ex, re/im) x coil group; columns past E
-    // write zeros, so every staged row holds 16 defined cells
-    constexpr int kU = CW / 8;
-    const int q = tid & 31, cg = tid >> 5;
-    const int ex = q >> 1, comp = q & 1;
+    // staging: thread = (cell column ex, coil), each cell's (re, im) read as
+    // one 8-byte word; columns past E write zeros, so every staged row holds
+    // 16 defined cells
+    constexpr int kU = CW / 16;
+    const int ex = tid & 15, cg = tid >> 4;
     const bool live = ex < E;
     int gx = sx + ex;
     gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
-    const long long cstride = 2 * L.sb * (long long)F;  // next coil
-    const float* gcol = grid + 2 * (long long)(live ? gx : 0) + comp + 2 * L.sb * (long long)f;
-    float v[E][kU];
+    const long long cstride = L.sb * (long long)F;  // next coil, in cells
+    const float2* gcol = reinterpret_cast<const float2*>(grid) + (long long)(live ? gx : 0) + L.sb * (long long)f;
+    float2 v[E][kU];
     float mloc = 0.0f;
 #pragma unroll
     for (int ey = 0; ey < E; ey++) {
         int gy = sy0 + ey;
         gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
-        const float* prow = gcol + 2 * L.sy * (long long)gy;
+        const float2* prow = gcol + L.sy * (long long)gy;
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            const int cl = cg + 8 * u;
-            v[ey][u] = (live && c0 + cl < C) ? prow[(long long)(c0 + cl) * cstride] : 0.0f;
+            const int cl = cg + 16 * u;
+            v[ey][u] = (live && c0 + cl < C) ? prow[(long long)(c0 + cl) * cstride] : make_float2(0.0f, 0.0f);
         }
     }
 #pragma unroll
@@ -829,8 +829,9 @@ __global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
         const float sgn = (sign && ((gx + gy) & 1)) ? -1.0f : 1.0f;
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            v[ey][u] *= sgn;
-            mloc = fmaxf(mloc, fabsf(v[ey][u]));
+            v[ey][u].x *= sgn;
+            v[ey][u].y *= sgn;
+            mloc = fmaxf(mloc, fmaxf(fabsf(v[ey][u].x), fabsf(v[ey][u].y)));
         }
     }
     const float wm = mf_wave_max(mloc);
@@ -845,12 +846,14 @@ __global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
     for (int ey = 0; ey < E; ey++)
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            const float x = v[ey][u] * R;
-            const _Float16 h = (_Float16)x;
-            const _Float16 l = (_Float16)(x - (float)h);
-            const int idx = (2 * (cg + 8 * u) + comp) * AS + ey * 16 + ex;  // row 2 coil + re/im
-            Ahi[idx] = h;
-            Alo[idx] = l;
+            const float xr = v[ey][u].x * R, xi = v[ey][u].y * R;
+            const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+            const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi);
+            const int idx = (2 * (cg + 16 * u)) * AS + ey * 16 + ex;  // rows 2 coil (re), 2 coil + 1 (im)
+            Ahi[idx] = hr;
+            Alo[idx] = lr;
+            Ahi[idx + AS] = hi;
+            Alo[idx + AS] = li;
         }
     __syncthreads();
 
