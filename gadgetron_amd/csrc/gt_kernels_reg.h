@@ -568,7 +568,7 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
     const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
     const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
     int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
-    int accumulate, float wscale) {
+    int accumulate, float wscale, int n_items, int nchunks) {
     static_assert(CW == 32 || CW == 16, "coils per wave");
     constexpr int LC = CW / 4;       // coils per lane in a chunk's sample loads
     constexpr int MB = 2 * CW / 32;  // 32-row blocks of (coil, re / im)
@@ -580,9 +580,14 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
     uint32_t* sh = reinterpret_cast<uint32_t*>(wT + Lds::Tab);  // [2][2 kNK] heads
     float* xs = reinterpret_cast<float*>(sh + Lds::Heads);      // [re / im][coil][XS] per entry
     const int lane = threadIdx.x;
-    const WorkItem it = items[blockIdx.x];
+    // block b = (item / 8) * 8 nchunks + chunk * 8 + item % 8: the coil chunks of an
+    // item run on its XCD (b mod 8 = item mod 8), 8 blocks apart, and share its L2
+    const int bx = (int)blockIdx.x, grp = 8 * nchunks;
+    const int item = (bx / grp) * 8 + (bx & 7), chunk = (bx / 8) % nchunks;
+    if (item >= n_items) return;
+    const WorkItem it = items[item];
     if (it.partial == kNoopItem) return;
-    const int c0 = blockIdx.y * CW;
+    const int c0 = chunk * CW;
     const long long FM = (long long)F * M;
     const uint32_t nch = (it.end - it.begin + kNK - 1) / kNK;
     const int kk = lane & (kNK - 1), cq = lane >> 4;  // sample loads: (entry, coil quarter)
@@ -781,7 +786,7 @@ template <int KW, int CW>
 __global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
     const float* __restrict__ grid, float* __restrict__ out, const float* __restrict__ wtab,
     const uint32_t* __restrict__ list, const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g,
-    Layout L, int sign, const float* __restrict__ dcw, long long dcw_len, int accumulate, float oscale) {
+    Layout L, int sign, const float* __restrict__ dcw, long long dcw_len, int accumulate, float oscale, int n_items, int nchunks) {
     constexpr int E = kRT + KW - 1;  // staged cells per dimension
     constexpr int AS = c2nc_mfma_row(KW);
     constexpr int MB = 2 * CW / 32;  // 32-row blocks
@@ -791,9 +796,14 @@ __global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
     _Float16* Alo = Ahi + 2 * CW * AS;
     float* red = reinterpret_cast<float*>(Alo + 2 * CW * AS);  // four wave maxima
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const WorkItem it = items[blockIdx.x];
+    // block b = (item / 8) * 8 nchunks + chunk * 8 + item % 8: the coil chunks of an
+    // item run on its XCD (b mod 8 = item mod 8), 8 blocks apart, and share its L2
+    const int bx = (int)blockIdx.x, grp = 8 * nchunks;
+    const int item = (bx / grp) * 8 + (bx & 7), chunk = (bx / 8) % nchunks;
+    if (item >= n_items) return;
+    const WorkItem it = items[item];
     if (it.partial == kNoopItem) return;
-    const int c0 = blockIdx.y * CW;
+    const int c0 = chunk * CW;
     const int fr = (int)floorf(g.radius);
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     const int t = (int)(it.key % (uint32_t)g.ntiles_total);

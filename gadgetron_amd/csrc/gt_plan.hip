@@ -686,10 +686,11 @@ struct LaunchFastNC2C {
                                         : (dcw_len == FM ? gt::nc2c_mfma2d_kernel<1, 16> : gt::nc2c_mfma2d_kernel<2, 16>))
                                 : (!dcw ? gt::nc2c_mfma2d_kernel<0, 32>
                                         : (dcw_len == FM ? gt::nc2c_mfma2d_kernel<1, 32> : gt::nc2c_mfma2d_kernel<2, 32>));
-            hipLaunchKernelGGL(km, dim3((unsigned)p->bn.n_items, mchunks), dim3(64), 0, s,
+            const unsigned nblk = (unsigned)((p->bn.n_items + 7) / 8 * 8) * mchunks;  // items x chunks, see the kernel
+            hipLaunchKernelGGL(km, dim3(nblk), dim3(64), 0, s,
                                samples, grid, partials, p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(),
                                p->bn.items.as<gt::WorkItem>(), int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw,
-                               dcw_len, accumulate, mfma_weight_scale(p));
+                               dcw_len, accumulate, mfma_weight_scale(p), p->bn.n_items, int(mchunks));
             return;
         }
         auto k = G == 4 ? pick<4>(dcw, dcw_len, FM) : (G == 2 ? pick<2>(dcw, dcw_len, FM) : pick<1>(dcw, dcw_len, FM));
@@ -721,10 +722,11 @@ struct LaunchFastC2NC {
             const int cwm = mfma_coils_per_wave();
             const unsigned mchunks = (unsigned)((C + cwm - 1) / cwm);
             auto km = cwm == 16 ? gt::c2nc_mfma2d_kernel<KW, 16> : gt::c2nc_mfma2d_kernel<KW, 32>;
-            hipLaunchKernelGGL(km, dim3((unsigned)b.n_items, mchunks), dim3(256),
-                               size_t(gt::c2nc_mfma_lds_bytes(KW, cwm)), s, grid, out, b.wtab.as<float>(),
-                               b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), int(p->M), int(p->F), C, b.geom, L,
-                               sign, dcw, dcw_len, accumulate, oscale);
+            const unsigned nblk = (unsigned)((b.n_items + 7) / 8 * 8) * mchunks;  // items x chunks, see the kernel
+            hipLaunchKernelGGL(km, dim3(nblk), dim3(256), size_t(gt::c2nc_mfma_lds_bytes(KW, cwm)), s, grid, out,
+                               b.wtab.as<float>(), b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), int(p->M),
+                               int(p->F), C, b.geom, L, sign, dcw, dcw_len, accumulate, oscale, b.n_items,
+                               int(mchunks));
             return;
         }
         const size_t lds = size_t(gt::c2nc_lds_bytes(KW, G));
