@@ -505,13 +505,14 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
 }
 
 // ---------------------------------------------------------------------------
-// nc2c_mfma2d_kernel: the 2D NC2C of 32 coils per wave on the matrix cores.
+// nc2c_mfma2d_kernel: the 2D NC2C of 16 or 32 coils per wave (CW) on the
+// matrix cores.
 //
 // A tile is a product of two matrices: grid[q][cell] = sum over entries k of
 // s[q][k] * w[k][cell], with q = (coil, re/im) and w[k][cell] = wr_k[y] * wc_k[x].
 // Each chunk of 16 table entries is one K = 16 step of
-// v_mfma_f32_32x32x16_f16 on four 32 x 32 blocks (re / im rows x upper /
-// lower half tile).  FP32 operands are split into fp16 pairs, x = hi + lo with
+// v_mfma_f32_32x32x16_f16 on 2 CW / 32 x 2 blocks of 32 x 32 (rows (re / im,
+// coil) x upper / lower half tile).  FP32 operands are split into fp16 pairs, x = hi + lo with
 // hi = fp16(x) and lo = fp16(x - hi), and each block takes the three products
 // hi*hi + hi*lo + lo*hi: every fp16 x fp16 product is exact in the FP32
 // accumulator, and the dropped lo*lo term and the two roundings leave a
@@ -762,15 +763,16 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// c2nc_mfma2d_kernel: the 2D C2NC of 32 coils on the matrix cores.
+// c2nc_mfma2d_kernel: the 2D C2NC of 16 or 32 coils per workgroup (CW) on
+// the matrix cores.
 //
 // The samples of a home tile are a product: s[q][n] = sum over staged cells k
 // of grid[q][k] * w[k][n], q = (coil, re/im), n = sample, w[k][n] = wy_n[y] *
 // wx_n[x].  A workgroup of four waves stages the tile + halo (E x E cells, E =
 // 8 + KW - 1, rows padded to 16 cells) once, scaled by a power of two R and
 // split into fp16 hi / lo planes in LDS; each wave then takes 32 samples and
-// walks the E staged rows as K = 16 steps of v_mfma_f32_32x32x16_f16 on two
-// 32-row blocks (coils 0-15, 16-31, rows 2 coil + re/im), with the three
+// walks the E staged rows as K = 16 steps of v_mfma_f32_32x32x16_f16 on
+// 2 CW / 32 blocks of 32 rows (16 coils each, rows 2 coil + re/im), with the three
 // products hi*hi + hi*lo + lo*hi as in nc2c_mfma2d_kernel.  The B operand is
 // the sample's weight row product, from a table that holds each sample's
 // weights at staged coordinates (c2nc_wtable2d_kernel).  Out: s * oscale / R

@@ -614,9 +614,9 @@ struct LaunchTablesC2NC {
     }
 };
 
-// The 2D NC2C of 32 coils per wave runs on the matrix cores unless
-// GT_NUFFT_MFMA=0 (the FP32 register-tile kernel, kept for comparison).
-bool nc2c_on_matrix_cores() {
+// The 2D gridding of many coils (NC2C from 9, C2NC from 17) runs on the
+// matrix cores unless GT_NUFFT_MFMA=0 (the FP32 kernels, kept for comparison).
+bool gridding_on_matrix_cores() {
     static const bool on = [] {
         const char* e = std::getenv("GT_NUFFT_MFMA");
         return !(e && e[0] == '0');
@@ -678,7 +678,7 @@ struct LaunchFastNC2C {
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
         const long long FM = (long long)(p->M * p->F);
         // 9 coils or more fill the 16-coil matrix-core kernel's rows at least half way
-        if ((G == 1 || (G == 2 && mfma_coils_per_wave() == 16)) && nc2c_on_matrix_cores()) {
+        if ((G == 1 || (G == 2 && mfma_coils_per_wave() == 16)) && gridding_on_matrix_cores()) {
             // the tile product on the matrix cores, CW coils per wave
             const int cwm = mfma_coils_per_wave();
             const unsigned mchunks = (unsigned)((C + cwm - 1) / cwm);
@@ -714,7 +714,7 @@ struct LaunchFastC2NC {
         const int G = coil_groups(C);
         const int cw = gt::kCoilsPerWave / G;
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
-        if (G == 1 && b.wtab.ptr && nc2c_on_matrix_cores()) {
+        if (G == 1 && b.wtab.ptr && gridding_on_matrix_cores()) {
             // 17 coils or more: the home tiles' products on the matrix cores,
             // cwm coils per workgroup (9 to 16 coils measured faster on the
             // FP32 gather: 20.6 against 22.7 us at 256^2, 16 coils)
@@ -1087,7 +1087,7 @@ size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& count
     }
     if (p->max_items_req > 0) return size_t(p->max_items_req);
     // the matrix-core NC2C spends less per entry: longer items, fewer partial tiles
-    if (p->fast2d) return nc2c_on_matrix_cores() ? 512 : 256;
+    if (p->fast2d) return gridding_on_matrix_cores() ? 512 : 256;
     if (p->fast3d) return 1024;
     const double avg = double(npairs) / double(std::max<size_t>(counts.size(), 1));
     return size_t(std::max(256.0, 4.0 * avg));
@@ -1137,8 +1137,8 @@ void build_c2nc(gt_nufft_plan* p, hipStream_t s) {
         b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
         dispatch_fast<LaunchTablesC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
                                         b.geom, b.table.as<uint32_t>(), s);
-        if (nc2c_on_matrix_cores()) {
-            // weight rows for the matrix-core C2NC (32 coils per wave)
+        if (gridding_on_matrix_cores()) {
+            // weight rows for the matrix-core C2NC (17 coils or more)
             b.wtab.reserve(std::max<size_t>(b.n, 1) * 32 * sizeof(float));
             const float sx = kb_axis_scale(p, 0), sy = kb_axis_scale(p, 1);
             if (b.n)
