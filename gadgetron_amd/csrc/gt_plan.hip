@@ -1086,8 +1086,16 @@ size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& count
         return p->max_items_req > 0 ? std::min(cap, size_t(p->max_items_req)) : cap;
     }
     if (p->max_items_req > 0) return size_t(p->max_items_req);
-    // the matrix-core NC2C spends less per entry: longer items, fewer partial tiles
-    if (p->fast2d) return gridding_on_matrix_cores() ? 512 : 256;
+    if (p->fast2d) {
+        // the matrix-core NC2C spends less per entry: items up to 512 entries
+        // (fewer partial tiles), but at least 2 items per CU for small problems
+        // (128^2, one frame of 16384 samples: 64-entry items 15 us, 512-entry 40 us)
+        const size_t cap = gridding_on_matrix_cores() ? 512 : 256;
+        const size_t target = npairs / size_t(2 * std::max(p->cus, 1));
+        size_t L = 64;
+        while (L * 2 <= std::min(cap, target)) L *= 2;
+        return L;
+    }
     if (p->fast3d) return 1024;
     const double avg = double(npairs) / double(std::max<size_t>(counts.size(), 1));
     return size_t(std::max(256.0, 4.0 * avg));
