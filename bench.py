@@ -136,32 +136,6 @@ def ndft_error(img, out, wl, rng, nsamp=512):
             "coil": 0, "reference": "FP64 direct NDFT / sqrt(G), times dcw"}
 
 
-def ellipse_phantom(rng, N, n=10):
-    """Sum of n random ellipses (Shepp-Logan-like) on an N x N grid."""
-    y, x = np.mgrid[-1:1:N * 1j, -1:1:N * 1j]
-    img = np.zeros((N, N), np.float64)
-    for _ in range(n):
-        cx, cy = rng.uniform(-0.6, 0.6, 2)
-        a, b = rng.uniform(0.05, 0.45, 2)
-        th = rng.uniform(0, np.pi)
-        xr = (x - cx) * np.cos(th) + (y - cy) * np.sin(th)
-        yr = -(x - cx) * np.sin(th) + (y - cy) * np.cos(th)
-        img[(xr / a) ** 2 + (yr / b) ** 2 <= 1.0] += rng.uniform(0.1, 1.0)
-    return img
-
-
-def gaussian_csm(rng, C, N):
-    """Normalised complex Gaussian bumps, sigma N/3, at random centres."""
-    y, x = np.mgrid[0:N, 0:N]
-    maps = []
-    for _ in range(C):
-        cy, cx = rng.uniform(0, N, 2)
-        maps.append(np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * (N / 3) ** 2)) * np.exp(1j * rng.uniform(0, 2 * np.pi)))
-    maps = np.array(maps)
-    maps /= np.sqrt(np.sum(np.abs(maps) ** 2, axis=0, keepdims=True))
-    return maps.astype(np.complex64)
-
-
 def batched_cgsense(nprob=64, iterations=10, reps=3, seed=3):
     """Batched uneven CG-SENSE: 64 independent 2D radial reconstructions,
     N in {96,128,160,192,256}, spokes U[32,512], 2N samples per spoke, coils
@@ -171,28 +145,21 @@ def batched_cgsense(nprob=64, iterations=10, reps=3, seed=3):
     import gadgetron_amd as ga
     from gadgetron_amd import DeviceArray
     from gadgetron_amd import trajectories as tj
-    rng = np.random.default_rng(seed)
     keep, ops, data, xp, xb = [], [], [], [], []
     work = 0
-    for _ in range(nprob):
-        N = int(rng.choice([96, 128, 160, 192, 256]))
-        spokes = int(rng.integers(32, 513))
-        C = int(rng.integers(4, 33))
-        traj = tj.golden_radial_2d(2 * N, spokes)
-        M = traj.shape[1]
+    for pr in tj.uneven_sense_problems(nprob, seed):
+        N, C, M = pr["N"], pr["C"], pr["M"]
         plan = ga.NFFTPlan([N, N], None, W=5.5, os_factor=1.5)
-        d_traj = DeviceArray.from_numpy(traj)
+        d_traj = DeviceArray.from_numpy(pr["traj"])
         plan.preprocess(d_traj)
-        d_csm = DeviceArray.from_numpy(gaussian_csm(rng, C, N))
-        d_dcw = DeviceArray.from_numpy(np.sqrt(tj.ramp_dcf(traj)).astype(np.float32))
+        d_csm = DeviceArray.from_numpy(pr["csm"])
+        d_dcw = DeviceArray.from_numpy(pr["dcw"])
         op = ga.SenseOperator(plan, d_csm, d_dcw)
-        d_x = DeviceArray.from_numpy(ellipse_phantom(rng, N).astype(np.complex64)[None])
+        d_x = DeviceArray.from_numpy(pr["x"])
         d_y = DeviceArray.zeros((C, 1, M), np.complex64)
         op.mult_M(d_x, d_y)
         plan.synchronize()
-        y = d_y.numpy()
-        y = (y + 1e-3 * (rng.standard_normal(y.shape) + 1j * rng.standard_normal(y.shape)) / np.sqrt(2)).astype(np.complex64)
-        d_data = DeviceArray.from_numpy(y)
+        d_data = DeviceArray.from_numpy((d_y.numpy() + pr["noise"]).astype(np.complex64))
         keep += [plan, d_traj, d_csm, d_dcw, d_x, d_y]
         ops.append(op)
         data.append(d_data)

@@ -52,6 +52,8 @@ EXPORTED_SYMBOLS = [
     "gt_cgsense_batch_run",
     "gt_cgsense_batch_destroy",
     "gt_cgsense_batch_branches",
+    "gt_image_operator_compute",
+    "gt_cgsense_precond_weights",
     # include/gt_radial.h
     "gt_radial_trajectory_golden_ratio_2d",
     "gt_radial_trajectory_fixed_angle_2d",
@@ -112,6 +114,14 @@ class gt_nufft_desc(ctypes.Structure):
     ]
 
 
+class gt_cg_reg(ctypes.Structure):
+    _fields_ = [
+        ("weight", ctypes.c_double),
+        ("image", ctypes.c_void_p),
+        ("image_len", ctypes.c_size_t),
+    ]
+
+
 class gt_cg_desc(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int),
@@ -120,6 +130,9 @@ class gt_cg_desc(ctypes.Structure):
         ("kappa", ctypes.c_double),
         ("precond", ctypes.c_void_p),
         ("precond_len", ctypes.c_size_t),
+        ("regs", ctypes.POINTER(gt_cg_reg)),
+        ("n_regs", ctypes.c_int),
+        ("x0", ctypes.c_void_p),
     ]
 
 
@@ -180,6 +193,8 @@ def lib():
         "gt_cgsense_batch_run": [P, P],
         "gt_cgsense_batch_destroy": [P],
         "gt_cgsense_batch_branches": [P, ctypes.POINTER(I)],
+        "gt_image_operator_compute": [P, S, I, FP, P, P],
+        "gt_cgsense_precond_weights": [P, S, S, P, D, P, P],
         "gt_radial_trajectory_golden_ratio_2d": [P, U, U, U, U, I, P],
         "gt_radial_trajectory_fixed_angle_2d": [P, U, U, U, ctypes.c_float, P],
         "gt_radial_trajectory_variable_angle_2d": [P, P, U, U, U, P],

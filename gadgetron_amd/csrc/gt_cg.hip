@@ -20,6 +20,7 @@
 #include <cmath>
 #include <memory>
 #include <numeric>
+#include <utility>
 #include <vector>
 
 #include "../../include/gt_cgsense.h"
@@ -89,10 +90,23 @@ struct CgParams {
     long long n;            // image elements
     float we_re;            // encoding operator weight
     float kappa;            // identity regularisation weight
+    const float* regd;      // or null: per element, kappa + sum_k weight_k image_k^2
     const float2* w;        // preconditioner weights or null
     long long wlen;
     double tol;
 };
+
+// Regulariser diagonal of the CG operator: out = (first ? base : out) +
+// weight * image^2 (image broadcast by modulus; null: the identity).
+__global__ void reg_diag_kernel(float* __restrict__ out, long long n, float base, const float* __restrict__ img,
+                                long long len, float weight, int first) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float w = 1.0f;
+        if (img) w = img[i % len];
+        const float t = weight * (w * w);  // imageOperator::mult_MH_M: x * image * image, then the weight
+        out[i] = (first ? base : out[i]) + t;
+    }
+}
 
 // Block reduction of a double2 in a fixed order.
 __device__ __forceinline__ double2 block_sum(double2 v, double2* sh) {
@@ -115,10 +129,12 @@ __device__ __forceinline__ float2 precond2(const CgParams& P, long long i, float
     return make_float2(t.x * w.x - t.y * w.y, t.x * w.y + t.y * w.x);
 }
 
-// Initialisation: r = rhs_scale * mhd; p = r (or P^2 r); partial sums of <r, p>.
+// Initialisation: r = rhs_scale * mhd; p = r (or P^2 r); x = 0 (x_mode 0),
+// x = r (1: what a solve of zero iterations returns) or x untouched (2: an
+// initial guess follows, which may be x itself); partial sums of <r, p>.
 __global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __restrict__ mhd, float2* __restrict__ x,
                                                               float2* __restrict__ r, float2* __restrict__ p,
-                                                              CgParams P, float rhs_scale,
+                                                              CgParams P, float rhs_scale, int x_mode,
                                                               double2* __restrict__ partial) {
     __shared__ double2 sh[kRedThreads];
     double2 acc = make_double2(0.0, 0.0);
@@ -128,7 +144,7 @@ __global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __re
         const float2 pv = P.w ? precond2(P, i, rv) : rv;
         r[i] = rv;
         p[i] = pv;
-        x[i] = make_float2(0.0f, 0.0f);
+        if (x_mode != 2) x[i] = x_mode ? rv : make_float2(0.0f, 0.0f);
         acc.x += (double)rv.x * pv.x + (double)rv.y * pv.y;  // Re(conj(r) p)
     }
     const double2 s = block_sum(acc, sh);
@@ -174,13 +190,65 @@ __global__ __launch_bounds__(kRedThreads) void cg_q_csm_kernel(const float2* __r
             a.y += s.x * v.y - s.y * v.x;
         }
         const float2 pv = p[i];
-        const float2 qv = make_float2(P.we_re * a.x + P.kappa * pv.x, P.we_re * a.y + P.kappa * pv.y);
+        const float kd = P.regd ? P.regd[i] : P.kappa;
+        const float2 qv = make_float2(P.we_re * a.x + kd * pv.x, P.we_re * a.y + kd * pv.y);
         q[i] = qv;
         acc.x += (double)pv.x * qv.x + (double)pv.y * qv.y;   // Re(conj(p) q)
         acc.y += (double)pv.x * qv.y - (double)pv.y * qv.x;   // Im(conj(p) q)
     }
     const double2 s = block_sum(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Initial guess (cgSolver::initialize with x0, cgSolver.h:254-277): x = x0,
+// r -= A x0 (A x0 = w_E sum_c conj(csm_c) img_c + regd x0, img = E^H E x0
+// before the coil sum), p = P^2 r; partial sums of <r, p>.
+__global__ __launch_bounds__(kRedThreads) void cg_x0_kernel(const float2* __restrict__ img,
+                                                            const float2* __restrict__ csm, long long nimg, int F,
+                                                            int C, const float2* __restrict__ x0,
+                                                            float2* __restrict__ x, float2* __restrict__ r,
+                                                            float2* __restrict__ p, CgParams P,
+                                                            double2* __restrict__ partial) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < P.n; i += (long long)gridDim.x * kRedThreads) {
+        const long long f = i / nimg, n = i - f * nimg;
+        float2 a = make_float2(0.0f, 0.0f);
+        for (int c = 0; c < C; c++) {
+            const float2 s = csm[n + nimg * c];
+            const float2 v = img[n + nimg * (f + (long long)F * c)];
+            a.x += s.x * v.x + s.y * v.y;
+            a.y += s.x * v.y - s.y * v.x;
+        }
+        const float2 xv = x0[i];
+        const float kd = P.regd ? P.regd[i] : P.kappa;
+        float2 rv = r[i];
+        rv.x -= P.we_re * a.x + kd * xv.x;
+        rv.y -= P.we_re * a.y + kd * xv.y;
+        const float2 pv = P.w ? precond2(P, i, rv) : rv;
+        r[i] = rv;
+        p[i] = pv;
+        x[i] = xv;
+        acc.x += (double)rv.x * pv.x + (double)rv.y * pv.y;
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// rq = Re <r, p> after the initial guess; rq_0 keeps the right-hand side's value.
+__global__ __launch_bounds__(kRedThreads) void cg_x0_finish_kernel(const double2* __restrict__ partial, int nb,
+                                                                   CgState* st) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) {
+        acc.x += partial[i].x;
+        acc.y += partial[i].y;
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        st->rq = s.x;
+        st->done = !(s.x > 0.0);  // x0 already solves the system
+    }
 }
 
 // alpha = rq / <p, q> (complex); zero once the solver has terminated.
@@ -346,6 +414,120 @@ __global__ void cg_p_kernel(float2* __restrict__ p, const float2* __restrict__ z
     }
 }
 
+// ---------------------------------------------------------------------------
+// imageOperator / cuImageOperator::compute (operators/imageOperator.h:28-47,
+// operators/gpu/cuImageOperator.h:30-63) and the CG-SENSE gadget's
+// preconditioner weights (gadgets/pmri/gpuCgSenseGadget.cpp:165-176)
+// ---------------------------------------------------------------------------
+constexpr int kHistBins = 50000;  // cuImageOperator::estimate_offset granularity
+
+// FP64 partial sums of |re| + |im| (asum, the BLAS definition for complex data).
+__global__ __launch_bounds__(kRedThreads) void asum_kernel(const float2* __restrict__ x, long long n,
+                                                           double2* __restrict__ partial) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kRedThreads) {
+        const float2 v = x[i];
+        acc.x += (double)fabsf(v.x) + (double)fabsf(v.y);
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// scal[0] = n / asum (float, as the reference's REAL arithmetic), scal[2] = asum.
+__global__ __launch_bounds__(kRedThreads) void asum_finish_kernel(const double2* __restrict__ partial, int nb,
+                                                                  long long n, float* __restrict__ scal) {
+    __shared__ double2 sh[kRedThreads];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) {
+        acc.x += partial[i].x;
+        acc.y += partial[i].y;
+    }
+    const double2 s = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        const float sum = (float)s.x;
+        scal[0] = (float)n / sum;
+        scal[2] = sum;
+    }
+}
+
+// out = |x * scale| (tmp *= scale; image_ = abs(tmp)); per-block maxima.
+__global__ __launch_bounds__(kRedThreads) void abs_scale_kernel(const float2* __restrict__ x, long long n,
+                                                                const float* __restrict__ scal, float* __restrict__ out,
+                                                                float* __restrict__ bmax) {
+    __shared__ float sh[kRedThreads];
+    const float scale = scal[0];
+    float m = 0.0f;
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kRedThreads) {
+        const float2 v = x[i];
+        const float re = v.x * scale, im = v.y * scale;
+        const float a = sqrtf(re * re + im * im);
+        out[i] = a;
+        m = fmaxf(m, a);
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int t = kRedThreads / 2; t > 0; t >>= 1) {
+        if ((int)threadIdx.x < t) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + t]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmax[blockIdx.x] = sh[0];
+}
+
+// scal[1] = the image maximum (amax of a non-negative image).
+__global__ __launch_bounds__(kRedThreads) void max_finish_kernel(const float* __restrict__ bmax, int nb,
+                                                                 float* __restrict__ scal) {
+    __shared__ float sh[kRedThreads];
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < nb; i += kRedThreads) m = fmaxf(m, bmax[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int t = kRedThreads / 2; t > 0; t >>= 1) {
+        if ((int)threadIdx.x < t) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + t]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) scal[1] = sh[0];
+}
+
+__global__ void zero_u32_kernel(unsigned int* __restrict__ y, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = 0u;
+}
+
+// bin = min(floor(value / max * 50000), 49999); integer counts (order free).
+__global__ void hist_kernel(const float* __restrict__ img, long long n, const float* __restrict__ scal,
+                            unsigned int* __restrict__ bins) {
+    const float mx = scal[1];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float b = floorf((img[i] / mx) * (float)kHistBins);
+        const unsigned int k = min((unsigned int)b, (unsigned int)(kHistBins - 1));
+        atomicAdd(&bins[k], 1u);
+    }
+}
+
+// image_ += offset (when positive); reciprocal_inplace.
+__global__ void offset_recip_kernel(float* __restrict__ img, long long n, float offset) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float v = img[i];
+        if (offset > 0.0f) v += offset;
+        img[i] = 1.0f / v;
+    }
+}
+
+// w = 1 / sqrt(sum_c |csm_c|^2 + kappa reg), as complex (w, 0).
+__global__ void precond_weights_kernel(const float2* __restrict__ csm, long long nimg, int C,
+                                       const float* __restrict__ reg, float kappa, float2* __restrict__ w) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nimg; i += (long long)gridDim.x * blockDim.x) {
+        float a = 0.0f;
+        for (int c = 0; c < C; c++) {
+            const float2 v = csm[i + nimg * c];
+            a += v.x * v.x + v.y * v.y;
+        }
+        if (reg) a += kappa * reg[i];
+        w[i] = make_float2(1.0f / sqrtf(a), 0.0f);
+    }
+}
+
 unsigned blocks_for(long long n, int threads) {
     return (unsigned)std::max<long long>(1, std::min<long long>((n + threads - 1) / threads, 1 << 20));
 }
@@ -369,8 +551,16 @@ struct gt_sense_op {
     DevBuf coil_smp;   // [M, F, C]
     // CG workspace
     DevBuf r, p, q, mhd;
+    DevBuf regd;  // regulariser diagonal (image regularisers)
     DevBuf partial, state, hist;
     bool warmed = false;
+    std::pair<uint64_t, uint64_t> warm_gen{0, 0};  // (plan, operator) generations at the last warm
+    // bumped by every reallocation of the buffers above (their epoch points
+    // here): a recorded batch graph holds their addresses
+    uint64_t generation = 0;
+    void track() {
+        for (DevBuf* b : {&coil_img, &coil_smp, &r, &p, &q, &mhd, &regd, &partial, &state, &hist}) b->epoch = &generation;
+    }
 
     size_t image_elems() const { return nimg * F; }
 
@@ -436,12 +626,13 @@ struct gt_sense_op {
         GT_HIP(hipGetLastError());
     }
 
-    void reserve_cg(int iterations) {
+    void reserve_cg(int iterations, bool regs = false) {
         const size_t bytes = image_elems() * sizeof(float2);
         r.reserve(bytes);
         p.reserve(bytes);
         q.reserve(bytes);
         mhd.reserve(bytes);
+        if (regs) regd.reserve(image_elems() * sizeof(float));
         partial.reserve(2 * kRedBlocks * sizeof(double2));
         state.reserve(size_t(iterations + 1) * sizeof(CgState));  // one per iteration (device-side solve)
         hist.reserve(size_t(iterations + 1) * sizeof(double));
@@ -449,11 +640,14 @@ struct gt_sense_op {
 
     // One E^H E application grows every buffer the solve uses (the plan's
     // grid, partial tiles, ...), so a later graph capture allocates nothing.
+    // It runs again whenever the plan or these buffers changed since.
     void warm(hipStream_t s) {
-        if (warmed) return;
+        const auto now = std::make_pair(plan_generation(plan), generation);
+        if (warmed && now == warm_gen) return;
         mult_MH_M(p.ptr, q.ptr, false, s);
         GT_HIP(hipStreamSynchronize(s));
         warmed = true;
+        warm_gen = std::make_pair(plan_generation(plan), generation);
     }
 };
 
@@ -464,6 +658,7 @@ CgParams cg_params(const gt_sense_op* op, const gt_cg_desc* d) {
     P.n = (long long)op->image_elems();
     P.we_re = float(d->encoding_weight);
     P.kappa = float(d->kappa);
+    P.regd = d->n_regs > 0 ? op->regd.as<float>() : nullptr;
     P.w = static_cast<const float2*>(d->precond);
     P.wlen = (long long)d->precond_len;
     P.tol = d->tc_tolerance;
@@ -474,6 +669,35 @@ void validate_desc(const gt_cg_desc* d) {
     GT_REQUIRE(d, GT_ERR_INVALID_ARGUMENT, "null CG descriptor");
     GT_REQUIRE(d->iterations >= 0, GT_ERR_INVALID_ARGUMENT, "negative iteration count");
     GT_REQUIRE(!d->precond || d->precond_len > 0, GT_ERR_INVALID_ARGUMENT, "preconditioner with zero length");
+    GT_REQUIRE(d->n_regs >= 0 && (d->n_regs == 0 || d->regs), GT_ERR_INVALID_ARGUMENT,
+               "regularisers: negative count or null list");
+    for (int k = 0; k < d->n_regs; k++)
+        GT_REQUIRE(!d->regs[k].image || d->regs[k].image_len > 0, GT_ERR_INVALID_ARGUMENT,
+                   "regulariser image with zero length");
+}
+
+// Sizes of the descriptor's arrays against the operator's image.
+void validate_sizes(const gt_sense_op* op, const gt_cg_desc* d) {
+    const long long n = (long long)op->image_elems();
+    GT_REQUIRE(!d->precond || n % (long long)d->precond_len == 0, GT_ERR_SHAPE,
+               "preconditioner length does not divide the image size");
+    for (int k = 0; k < d->n_regs; k++)
+        GT_REQUIRE(!d->regs[k].image || n % (long long)d->regs[k].image_len == 0, GT_ERR_SHAPE,
+                   "regulariser image length does not divide the image size");
+}
+
+// regd = kappa + sum_k weight_k image_k^2, in list order (cgSolver::mult_MH_M
+// adds the regularisers after the encoding operator, cgSolver.h:376-384).
+void build_regd(gt_sense_op* op, const gt_cg_desc* d, hipStream_t s) {
+    if (d->n_regs <= 0) return;
+    const long long n = (long long)op->image_elems();
+    for (int k = 0; k < d->n_regs; k++) {
+        const gt_cg_reg& g = d->regs[k];
+        hipLaunchKernelGGL(reg_diag_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, op->regd.as<float>(), n,
+                           float(d->kappa), g.image, (long long)std::max<size_t>(g.image_len, 1), float(g.weight),
+                           k == 0 ? 1 : 0);
+    }
+    GT_HIP(hipGetLastError());
 }
 
 // Host callback of the plain solver: the reference reads every dot product
@@ -490,9 +714,9 @@ void host_read(const gt_sense_op* op, CgState* h, hipStream_t s) {
 // of samples whose w_E E^H image is the right-hand side.
 void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, bool host_sync, hipStream_t s,
               bool image_rhs = false) {
+    validate_sizes(op, d);
     const CgParams P = cg_params(op, d);
-    GT_REQUIRE(!P.w || (long long)op->image_elems() % P.wlen == 0, GT_ERR_SHAPE,
-               "preconditioner length does not divide the image size");
+    build_regd(op, d, s);
     const unsigned nb = (unsigned)std::min<long long>(kRedBlocks, (P.n + kRedThreads - 1) / kRedThreads);
     double2* part = op->partial.as<double2>();
     CgState* st = op->state.as<CgState>();
@@ -505,9 +729,19 @@ void cg_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* d, b
     else
         op->mult_MH(data, op->mhd.ptr, false, s);
     hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->mhd.as<float2>(), xf,
-                       op->r.as<float2>(), op->p.as<float2>(), P, image_rhs ? 1.0f : P.we_re, part);
+                       op->r.as<float2>(), op->p.as<float2>(), P, image_rhs ? 1.0f : P.we_re,
+                       d->iterations == 0 ? 1 : (d->x0 ? 2 : 0), part);
     hipLaunchKernelGGL(cg_init_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st, hist);
     GT_HIP(hipGetLastError());
+    if (d->x0 && d->iterations > 0) {
+        // x = x0, r = rhs - A x0, p = P^2 r (rq_0 stays the right-hand side's)
+        op->mult_MH_M_coils(d->x0, s);
+        hipLaunchKernelGGL(cg_x0_kernel, dim3(nb), dim3(kRedThreads), 0, s, op->coil_img.as<float2>(), op->csm,
+                           (long long)op->nimg, int(op->F), int(op->C), static_cast<const float2*>(d->x0), xf,
+                           op->r.as<float2>(), op->p.as<float2>(), P, part);
+        hipLaunchKernelGGL(cg_x0_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, part, int(nb), st);
+        GT_HIP(hipGetLastError());
+    }
     if (!host_sync) {
         // device-side: iteration it reads state it and writes state it + 1
         for (int it = 0; it < d->iterations; it++) {
@@ -556,11 +790,16 @@ struct gt_cg_batch {
     std::vector<const void*> data;
     std::vector<void*> x;
     gt_cg_desc desc{};
+    std::vector<gt_cg_reg> regs;  // desc.regs points here (the caller's list may go away)
     int mode = GT_CG_PLAIN;
     int device = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int branches = 0;  // chosen by tune_graph
+    // generations of each problem's plan and operator when the graph was
+    // recorded: a replay after either changed would use stale addresses and
+    // launch sizes
+    std::vector<std::pair<uint64_t, uint64_t>> gens;
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> events;
     ~gt_cg_batch() {
@@ -673,6 +912,7 @@ gt_status gt_sense_create(gt_sense_op** out, gt_nufft_plan* plan, const void* cs
         GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
         GT_REQUIRE(!plan_is_double(plan), GT_ERR_UNSUPPORTED, "the SENSE operator takes float (GT_FLOAT) plans");
         auto op = std::make_unique<gt_sense_op>();
+        op->track();
         op->plan = plan;
         size_t N[4] = {0, 0, 0, 0};
         auto chk = [](gt_status s) {
@@ -733,7 +973,7 @@ gt_status gt_cgsense_solve(gt_sense_op* op, const void* data, void* x, const gt_
         GT_REQUIRE(op && data && x, GT_ERR_INVALID_ARGUMENT, "null argument");
         validate_desc(desc);
         hipStream_t s = S(stream);
-        op->reserve_cg(desc->iterations);
+        op->reserve_cg(desc->iterations, desc->n_regs > 0);
         cg_solve(op, data, x, desc, false, s);
         if (rq_history) {
             GT_HIP(hipMemcpyAsync(rq_history, op->hist.ptr, size_t(desc->iterations + 1) * sizeof(double),
@@ -749,7 +989,7 @@ gt_status gt_cgsense_solve_image(gt_sense_op* op, const void* rhs, void* x, cons
         GT_REQUIRE(op && rhs && x, GT_ERR_INVALID_ARGUMENT, "null argument");
         validate_desc(desc);
         hipStream_t s = S(stream);
-        op->reserve_cg(desc->iterations);
+        op->reserve_cg(desc->iterations, desc->n_regs > 0);
         cg_solve(op, rhs, x, desc, false, s, true);
         if (rq_history) {
             GT_HIP(hipMemcpyAsync(rq_history, op->hist.ptr, size_t(desc->iterations + 1) * sizeof(double),
@@ -766,8 +1006,16 @@ gt_status gt_cgsense_batch_create(gt_cg_batch** out, gt_sense_op* const* ops, co
         *out = nullptr;
         validate_desc(desc);
         GT_REQUIRE(mode == GT_CG_PLAIN || mode == GT_CG_BATCHED, GT_ERR_INVALID_ARGUMENT, "unknown batch mode");
+        // one descriptor serves problems of different sizes: no initial guess
+        // or regulariser images (a preconditioner must divide every image)
+        GT_REQUIRE(!desc->x0, GT_ERR_UNSUPPORTED, "a batch takes no initial guess");
+        for (int k = 0; k < desc->n_regs; k++)
+            GT_REQUIRE(!desc->regs[k].image, GT_ERR_UNSUPPORTED,
+                       "a batch takes no regulariser images (one descriptor serves problems of different sizes)");
         auto b = std::make_unique<gt_cg_batch>();
         b->desc = *desc;
+        b->regs.assign(desc->regs, desc->regs + desc->n_regs);
+        b->desc.regs = b->regs.empty() ? nullptr : b->regs.data();
         b->mode = mode;
         GT_HIP(hipGetDevice(&b->device));
         hipStream_t s = S(stream);
@@ -776,13 +1024,17 @@ gt_status gt_cgsense_batch_create(gt_cg_batch** out, gt_sense_op* const* ops, co
             for (int j = 0; j < i; j++)
                 GT_REQUIRE(ops[j] != ops[i] && ops[j]->plan != ops[i]->plan, GT_ERR_INVALID_ARGUMENT,
                            "problems of one batch need their own operator and plan (the branches run concurrently)");
-            ops[i]->reserve_cg(desc->iterations);
+            validate_sizes(ops[i], desc);
+            ops[i]->reserve_cg(desc->iterations, desc->n_regs > 0);
             ops[i]->warm(s);
             b->ops.push_back(ops[i]);
             b->data.push_back(data[i]);
             b->x.push_back(x[i]);
         }
-        if (mode == GT_CG_BATCHED) tune_graph(b.get(), s);
+        if (mode == GT_CG_BATCHED) {
+            tune_graph(b.get(), s);
+            for (auto* o : b->ops) b->gens.emplace_back(plan_generation(o->plan), o->generation);
+        }
         *out = b.release();
     });
 }
@@ -792,10 +1044,78 @@ gt_status gt_cgsense_batch_run(gt_cg_batch* b, gt_stream_t stream) {
         GT_REQUIRE(b, GT_ERR_INVALID_ARGUMENT, "null batch");
         hipStream_t s = S(stream);
         if (b->mode == GT_CG_BATCHED) {
+            for (size_t i = 0; i < b->ops.size(); i++)
+                GT_REQUIRE(b->gens[i].first == plan_generation(b->ops[i]->plan) &&
+                               b->gens[i].second == b->ops[i]->generation,
+                           GT_ERR_INVALID_ARGUMENT,
+                           "a plan or operator bound to this batch changed after gt_cgsense_batch_create "
+                           "(preprocess, set_tiling or a workspace reallocation): create the batch again");
             GT_HIP(hipGraphLaunch(b->exec, s));
         } else {
             for (size_t i = 0; i < b->ops.size(); i++) cg_solve(b->ops[i], b->data[i], b->x[i], &b->desc, true, s);
         }
+    });
+}
+
+gt_status gt_image_operator_compute(const void* image, size_t n, int offset_estimation, float* offset, float* out,
+                                    gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(image && offset && out, GT_ERR_INVALID_ARGUMENT, "null argument");
+        GT_REQUIRE(n > 0 && n < (size_t(1) << 31), GT_ERR_INVALID_ARGUMENT, "image size must be in [1, 2^31)");
+        hipStream_t s = S(stream);
+        const long long nn = (long long)n;
+        const unsigned nb = (unsigned)std::min<long long>(kRedBlocks, (nn + kRedThreads - 1) / kRedThreads);
+        DevBuf part, scal, bmax, bins;
+        part.reserve(nb * sizeof(double2));
+        scal.reserve(4 * sizeof(float));
+        bmax.reserve(nb * sizeof(float));
+        const float2* x = static_cast<const float2*>(image);
+        hipLaunchKernelGGL(asum_kernel, dim3(nb), dim3(kRedThreads), 0, s, x, nn, part.as<double2>());
+        hipLaunchKernelGGL(asum_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, part.as<double2>(), int(nb), nn,
+                           scal.as<float>());
+        hipLaunchKernelGGL(abs_scale_kernel, dim3(nb), dim3(kRedThreads), 0, s, x, nn, scal.as<float>(), out,
+                           bmax.as<float>());
+        hipLaunchKernelGGL(max_finish_kernel, dim3(1), dim3(kRedThreads), 0, s, bmax.as<float>(), int(nb),
+                           scal.as<float>());
+        GT_HIP(hipGetLastError());
+        float h[4] = {0, 0, 0, 0};
+        GT_HIP(hipMemcpyAsync(h, scal.ptr, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        GT_REQUIRE(h[2] > 0.0f && std::isfinite(h[0]), GT_ERR_INVALID_ARGUMENT,
+                   "the regularisation image is zero (its normalisation divides by sum |x|)");
+        if (offset_estimation) {
+            // cuImageOperator::estimate_offset: histogram of image / max in
+            // 50000 bins; the offset is (bins to reach 1 % of the elements + 1)
+            // times max / 50000
+            bins.reserve(kHistBins * sizeof(unsigned int));
+            hipLaunchKernelGGL(zero_u32_kernel, dim3(blocks_for(kHistBins, 256)), dim3(256), 0, s,
+                               bins.as<unsigned int>(), (long long)kHistBins);
+            hipLaunchKernelGGL(hist_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, s, out, nn, scal.as<float>(),
+                               bins.as<unsigned int>());
+            GT_HIP(hipGetLastError());
+            std::vector<unsigned int> hist(kHistBins);
+            GT_HIP(hipMemcpyAsync(hist.data(), bins.ptr, kHistBins * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+            GT_HIP(hipStreamSynchronize(s));
+            const unsigned int target = (unsigned int)(0.01f * (float)n);
+            unsigned int cumsum = 0, counter = 0;
+            while (cumsum < target) cumsum += hist[counter++];
+            *offset = float(counter + 1) * h[1] / float(kHistBins);
+        }
+        hipLaunchKernelGGL(offset_recip_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, s, out, nn, *offset);
+        GT_HIP(hipGetLastError());
+        GT_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+    });
+}
+
+gt_status gt_cgsense_precond_weights(const void* csm, size_t nimg, size_t ncoils, const float* reg, double kappa,
+                                     void* weights, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(csm && weights, GT_ERR_INVALID_ARGUMENT, "null argument");
+        GT_REQUIRE(nimg > 0 && ncoils > 0, GT_ERR_INVALID_ARGUMENT, "empty image or no coils");
+        hipLaunchKernelGGL(precond_weights_kernel, dim3(blocks_for((long long)nimg, 256)), dim3(256), 0, S(stream),
+                           static_cast<const float2*>(csm), (long long)nimg, int(ncoils), reg, float(kappa),
+                           static_cast<float2*>(weights));
+        GT_HIP(hipGetLastError());
     });
 }
 

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <new>
 #include <sstream>
 #include <stdexcept>
@@ -58,12 +59,16 @@ gt_status guarded(F&& f) {
 
 inline hipStream_t S(gt_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Grow-only device buffer.
+// Grow-only device buffer.  `epoch`, when set, is the owner's generation
+// counter: every reallocation bumps it, so work recorded against the old
+// address (a captured hipGraph) can tell that it is stale.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    uint64_t* epoch = nullptr;
     void reserve(size_t n) {
         if (n <= bytes) return;
+        if (epoch) ++*epoch;
         if (ptr) GT_HIP(hipFree(ptr));
         ptr = nullptr;
         bytes = 0;
@@ -78,7 +83,7 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes), epoch(o.epoch) {
         o.ptr = nullptr;
         o.bytes = 0;
     }
@@ -96,5 +101,8 @@ bool forward_with_coil_maps(gt_nufft_plan* p, const void* x, const void* csm, si
                             const float* dcw, size_t dcw_len, hipStream_t s);
 // True for a GT_DOUBLE plan (the SENSE operator and CG solver take float plans).
 bool plan_is_double(const gt_nufft_plan* p);
+// The plan's generation: it changes whenever preprocess, set_tiling or a
+// workspace reallocation moves or rewrites state that launched kernels read.
+uint64_t plan_generation(const gt_nufft_plan* p);
 
 }  // namespace gt_host

@@ -734,6 +734,13 @@ __global__ __launch_bounds__(64 * kRowsPerBlock) void crop_deapod_kernel(const f
     }
 }
 
+// y = 0 over n complex values, grid-stride: clears a grid inside code a caller
+// may record into a hipGraph (no memset node there).
+__global__ __launch_bounds__(256) void zero_kernel(float2* __restrict__ y, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = make_float2(0.0f, 0.0f);
+}
+
 // y += x, grid-stride (gt_array_add).
 template <class T>
 __global__ __launch_bounds__(256) void add_kernel(T* __restrict__ y, const T* __restrict__ x, long long n) {

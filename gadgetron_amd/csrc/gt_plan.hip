@@ -165,6 +165,10 @@ struct gt_nufft_plan {
     // preprocess state
     size_t M = 0, F = 0;
     bool prep_c2nc = false, prep_nc2c = false;
+    // generation: bumped by preprocess, set_tiling and every reallocation of
+    // a buffer below (their epoch points here), so a recorded launch sequence
+    // can detect that it refers to stale state
+    uint64_t generation = 0;
     DevBuf traj;  // scaled positions [M*F][D] (GT_DOUBLE: a float copy for the binning)
     DevBuf traj_d;  // GT_DOUBLE: scaled positions in double
     DevBuf row_inv;     // SPARSE_MATRIX: 1 / window weight sum per sample [M*F]
@@ -223,6 +227,15 @@ struct gt_nufft_plan {
         grid.release();
         tmp_samples.release();
         partials.release();
+    }
+
+    // points every workspace and table buffer's epoch at `generation`
+    void track() {
+        for (DevBuf* b : {&traj, &traj_d, &row_inv, &sparse_dcw, &flag, &grid, &tmp_samples, &partials})
+            b->epoch = &generation;
+        for (Binning* bb : {&bn, &bc})
+            for (DevBuf* b : {&bb->list, &bb->table, &bb->heads, &bb->dcw_e, &bb->wtab, &bb->items, &bb->red})
+                b->epoch = &generation;
     }
 
     size_t G() const {
@@ -1223,8 +1236,10 @@ void conv_c2nc(gt_nufft_plan* p, const void* grid, const gt::Layout& L, void* ou
     GT_HIP(hipGetLastError());
 }
 
+// partials_ext: where the split tiles' partial sums go instead of the plan's
+// workspace (the caller has checked its size); null: the plan's, grown here.
 void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layout& L, size_t C, int sign,
-               const float* dcw, size_t dcw_len, bool accumulate, hipStream_t s) {
+               const float* dcw, size_t dcw_len, bool accumulate, hipStream_t s, void* partials_ext = nullptr) {
     GT_REQUIRE(p->prep_nc2c, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for NC2C");
     Binning& b = p->bn;
     if (p->conv_type == GT_CONV_SPARSE_MATRIX) {
@@ -1244,51 +1259,57 @@ void conv_nc2c(gt_nufft_plan* p, const void* samples, void* grid, const gt::Layo
             dcw_len = size_t(n);
         }
     }
-    if (p->conv_type != GT_CONV_ATOMIC && b.n_partials > 0)
+    if (p->conv_type != GT_CONV_ATOMIC && b.n_partials > 0 && !partials_ext)
         p->partials.reserve(size_t(b.n_partials) * C * b.geom.tile_cells * sizeof(float2));
+    float* part = partials_ext ? static_cast<float*>(partials_ext) : p->partials.as<float>();
     StageTimer timer(p, GT_STAGE_NC2C, s);
     const long long dl = (long long)std::max<size_t>(dcw_len, 1);
     if (p->conv_type == GT_CONV_ATOMIC) {
-        if (!accumulate) GT_HIP(hipMemsetAsync(grid, 0, p->G() * p->F * C * sizeof(float2), s));
+        if (!accumulate) {
+            // a fill kernel, not a memset: SENSE / CG solves over this plan are
+            // recorded into hipGraphs
+            const long long nz = (long long)(p->G() * p->F * C);
+            hipLaunchKernelGGL(gt::zero_kernel, dim3(grid_blocks(nz)), dim3(256), 0, s, (float2*)grid, nz);
+        }
         dispatch4<LaunchNC2CAtomic>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->traj.as<float>(),
                                    int(p->M), int(p->F), int(C), p->geom, sign, dcw, dl, s);
         GT_HIP(hipGetLastError());
         return;
     }
     if (p->fast2d) {
-        dispatch_fast<LaunchFastNC2C>(p->KW, (const float*)samples, (float*)grid, p->partials.as<float>(), p,
+        dispatch_fast<LaunchFastNC2C>(p->KW, (const float*)samples, (float*)grid, part, p,
                                       int(C), L, sign, dcw, dl, accumulate ? 1 : 0, s);
         GT_HIP(hipGetLastError());
         if (b.n_red > 0) {
             const unsigned parts = (unsigned)((C * gt::kRT * gt::kRT * 2 + 255) / 256);
             hipLaunchKernelGGL(gt::nc2c_reduce2d_kernel, dim3((unsigned)b.n_red, parts), dim3(256), 0, s,
-                               p->partials.as<float>(), (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                               part, (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
                                b.geom, L, sign, accumulate ? 1 : 0);
             GT_HIP(hipGetLastError());
         }
         return;
     }
     if (p->fast3d) {
-        launch_fast3_nc2c((const float*)samples, (float*)grid, p->partials.as<float>(), p, int(C), L, sign, dcw,
+        launch_fast3_nc2c((const float*)samples, (float*)grid, part, p, int(C), L, sign, dcw,
                           dl, accumulate ? 1 : 0, s);
         GT_HIP(hipGetLastError());
         if (b.n_red > 0) {
             const unsigned parts = (unsigned)((C * b.geom.tile_cells * 2 + 255) / 256);
             hipLaunchKernelGGL(gt::nc2c_reduce3d_kernel, dim3((unsigned)b.n_red, parts), dim3(256), 0, s,
-                               p->partials.as<float>(), (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                               part, (float*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
                                b.geom, L, sign, accumulate ? 1 : 0);
             GT_HIP(hipGetLastError());
         }
         return;
     }
     const int CC = coil_chunk(p);
-    dispatch4<LaunchNC2CTile>(p->D, p->KW, (const float2*)samples, (float2*)grid, p->partials.as<float2>(),
+    dispatch4<LaunchNC2CTile>(p->D, p->KW, (const float2*)samples, (float2*)grid, reinterpret_cast<float2*>(part),
                              p->traj.as<float>(), b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), b.n_items,
                              int(p->M), int(p->F), int(C), CC, b.geom, sign, dcw, dl, accumulate ? 1 : 0, s);
     GT_HIP(hipGetLastError());
     if (b.n_red > 0) {
         hipLaunchKernelGGL(gt::nc2c_reduce_kernel, dim3((unsigned)b.n_red), dim3(256), 0, s,
-                           p->partials.as<float2>(), (float2*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                           reinterpret_cast<float2*>(part), (float2*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
                            b.geom, sign, accumulate ? 1 : 0);
         GT_HIP(hipGetLastError());
     }
@@ -1369,22 +1390,23 @@ void conv_c2nc_d(gt_nufft_plan* p, const void* grid, void* out, size_t C, int si
 }
 
 void conv_nc2c_d(gt_nufft_plan* p, const void* samples, void* grid, size_t C, int sign, const double* dcw,
-                 size_t dcw_len, bool accumulate, hipStream_t s) {
+                 size_t dcw_len, bool accumulate, hipStream_t s, void* partials_ext = nullptr) {
     GT_REQUIRE(p->prep_nc2c, GT_ERR_NOT_PREPROCESSED, "plan not preprocessed for NC2C");
     Binning& b = p->bn;
     const int TC = std::max(1, b.geom.tile_cells);
-    if (b.n_partials > 0) p->partials.reserve(size_t(b.n_partials) * C * TC * sizeof(double2));
+    if (b.n_partials > 0 && !partials_ext) p->partials.reserve(size_t(b.n_partials) * C * TC * sizeof(double2));
+    double2* part = partials_ext ? static_cast<double2*>(partials_ext) : p->partials.as<double2>();
     StageTimer timer(p, GT_STAGE_NC2C, s);
     // LDS budget per one-wave workgroup: 16 KiB of re/im accumulators
     const int CC = std::max(1, std::min(16384 / (16 * TC), 32));
     const long long dl = (long long)std::max<size_t>(dcw_len, 1);
-    dispatch4<LaunchNC2CTileD>(p->D, p->KW, (const double2*)samples, (double2*)grid, p->partials.as<double2>(),
+    dispatch4<LaunchNC2CTileD>(p->D, p->KW, (const double2*)samples, (double2*)grid, part,
                               p->traj_d.as<double>(), b.list.as<uint32_t>(), b.items.as<gt::WorkItem>(), b.n_items,
                               int(p->M), int(p->F), int(C), CC, b.geom, sign, dcw, dl, accumulate ? 1 : 0, s);
     GT_HIP(hipGetLastError());
     if (b.n_red > 0) {
         hipLaunchKernelGGL(gt::nc2c_reduce_d_kernel, dim3((unsigned)b.n_red), dim3(256), 0, s,
-                           p->partials.as<double2>(), (double2*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
+                           part, (double2*)grid, b.red.as<gt::RedItem>(), int(p->F), int(C),
                            b.geom, sign, accumulate ? 1 : 0);
         GT_HIP(hipGetLastError());
     }
@@ -1641,6 +1663,7 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
 }  // namespace
 
 bool gt_host::plan_is_double(const gt_nufft_plan* p) { return p->dbl; }
+uint64_t gt_host::plan_generation(const gt_nufft_plan* p) { return p->generation; }
 
 bool gt_host::forward_with_coil_maps(gt_nufft_plan* p, const void* x, const void* csm, size_t C, void* out,
                                      const float* dcw, size_t dcw_len, hipStream_t s) {
@@ -1683,6 +1706,7 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         GT_REQUIRE(d->os_factor == 0.0 || d->os_factor >= 1.0, GT_ERR_INVALID_ARGUMENT,
                    "Oversampling factor must greater than or equal to 1.");
         auto p = std::make_unique<gt_nufft_plan>();
+        p->track();
         p->D = d->D;
         p->W = d->W;
         p->Wf = float(d->W);
@@ -1775,6 +1799,7 @@ gt_status gt_nufft_set_tiling(gt_nufft_plan* p, const int* tile, int max_samples
             p->tile_req[d] = t;
         }
         p->max_items_req = max_samples_per_item;
+        p->generation++;
         setup_geometry(p);
         int cells = p->geom.tile_cells;
         GT_REQUIRE(size_t(2) * 4 * cells <= 65536, GT_ERR_INVALID_ARGUMENT, "tile does not fit in LDS");
@@ -1794,6 +1819,7 @@ gt_status gt_nufft_preprocess(gt_nufft_plan* p, const void* traj, size_t M, size
         GT_REQUIRE(M * F < (size_t(1) << 31), GT_ERR_UNSUPPORTED, "too many samples");
         hipStream_t s = S(stream);
         set_device(p);
+        p->generation++;
         p->prep_c2nc = p->prep_nc2c = false;
         p->M = p->F = 0;
         const long long n = (long long)(M * F * p->D);

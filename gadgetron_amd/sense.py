@@ -10,10 +10,10 @@ import ctypes
 
 import numpy as np
 
-from ._lib import GTError, check, gt_cg_desc, lib
+from ._lib import GTError, check, gt_cg_desc, gt_cg_reg, lib
 from .nfft import NFFTPlan, _numel, _ptr, _stream
 
-__all__ = ["SenseOperator", "cg_sense", "cg_sense_image", "CgBatch"]
+__all__ = ["SenseOperator", "cg_sense", "cg_sense_image", "CgBatch", "ImageOperator", "precond_weights"]
 
 
 class SenseOperator:
@@ -54,7 +54,10 @@ class SenseOperator:
         check(lib().gt_sense_mult_MH_M(self._h, _ptr(x), _ptr(out), int(bool(accumulate)), _stream(stream)))
 
 
-def _desc(iterations, tc_tolerance, encoding_weight, kappa, precond):
+def _desc(iterations, tc_tolerance, encoding_weight, kappa, precond, regs=None, x0=None):
+    """regs: None or a list of (weight, image) image regularisers (image None:
+    the identity; otherwise a float32 device array).  The returned descriptor
+    keeps the regulariser list alive as d._regs."""
     d = gt_cg_desc()
     d.iterations = int(iterations)
     d.tc_tolerance = float(tc_tolerance)
@@ -62,25 +65,38 @@ def _desc(iterations, tc_tolerance, encoding_weight, kappa, precond):
     d.kappa = float(kappa)
     d.precond = None if precond is None else int(precond.data_ptr())
     d.precond_len = 0 if precond is None else _numel(precond)
+    regs = list(regs or [])
+    arr = (gt_cg_reg * max(1, len(regs)))()
+    for k, (w, img) in enumerate(regs):
+        arr[k].weight = float(w)
+        arr[k].image = None if img is None else int(img.data_ptr())
+        arr[k].image_len = 0 if img is None else _numel(img)
+    d._regs = arr
+    d.regs = ctypes.cast(arr, ctypes.POINTER(gt_cg_reg)) if regs else None
+    d.n_regs = len(regs)
+    d.x0 = None if x0 is None else int(x0.data_ptr())
     return d
 
 
 def cg_sense(op: SenseOperator, data, x, iterations=10, tc_tolerance=1e-6, encoding_weight=1.0, kappa=0.0,
-             precond=None, stream=None):
+             precond=None, regs=None, x0=None, stream=None):
     """cgSolver::solve(data) into x; returns rq_0 and rq after each iteration
     (zeros after termination).  precond: complex64 diagonal weights (applied
-    twice, as cgPreconditioner::apply in the reference recurrence)."""
-    d = _desc(iterations, tc_tolerance, encoding_weight, kappa, precond)
+    twice, as cgPreconditioner::apply in the reference recurrence); regs: image
+    regularisers (weight, real image) added as weight * image^2 * x
+    (imageOperator::mult_MH_M); x0: initial guess (may be x itself).
+    iterations 0 returns the right-hand side, as cgSolver::solve_from_rhs."""
+    d = _desc(iterations, tc_tolerance, encoding_weight, kappa, precond, regs, x0)
     hist = (ctypes.c_double * (int(iterations) + 1))()
     check(lib().gt_cgsense_solve(op._h, _ptr(data), _ptr(x), ctypes.byref(d), hist, _stream(stream)))
     return np.array(hist[:], dtype=np.float64)
 
 
 def cg_sense_image(op: SenseOperator, rhs, x, iterations=10, tc_tolerance=1e-6, encoding_weight=1.0, kappa=0.0,
-                   precond=None, stream=None):
+                   precond=None, regs=None, x0=None, stream=None):
     """The solver of cg_sense with an image-space right-hand side:
-    x = CG(w_E E^H E + kappa I, rhs); returns the residual history."""
-    d = _desc(iterations, tc_tolerance, encoding_weight, kappa, precond)
+    x = CG(w_E E^H E + kappa I + regularisers, rhs); returns the residual history."""
+    d = _desc(iterations, tc_tolerance, encoding_weight, kappa, precond, regs, x0)
     hist = (ctypes.c_double * (int(iterations) + 1))()
     check(lib().gt_cgsense_solve_image(op._h, _ptr(rhs), _ptr(x), ctypes.byref(d), hist, _stream(stream)))
     return np.array(hist[:], dtype=np.float64)
@@ -127,3 +143,43 @@ class CgBatch:
                 self._h = None
         except Exception:
             pass
+
+
+class ImageOperator:
+    """cuImageOperator<float_complext> (operators/imageOperator.h,
+    operators/gpu/cuImageOperator.h): a regulariser whose real image is
+    1 / (|x| n / asum(x) + offset), computed from a complex image x; the
+    offset comes from the reference's 50000-bin histogram (the 1 % quantile)
+    or is given.  `image` is a float32 device array of x's size; `offset` the
+    offset used.  As a CG regulariser it contributes weight * image^2 * x
+    (pass (weight, op.image) in cg_sense's regs)."""
+
+    def __init__(self, weight=1.0):
+        self.weight = float(weight)
+        self.image = None
+        self.offset = 0.0
+
+    def compute(self, x, offset_estimation=True, stream=None):
+        from .nfft import DeviceArray
+        n = _numel(x)
+        if self.image is None or _numel(self.image) != n:
+            self.image = DeviceArray.zeros(tuple(int(v) for v in x.shape), np.float32)
+        off = ctypes.c_float(self.offset)
+        check(lib().gt_image_operator_compute(_ptr(x), n, int(bool(offset_estimation)), ctypes.byref(off),
+                                              _ptr(self.image), _stream(stream)))
+        self.offset = float(off.value)
+        return self.image
+
+
+def precond_weights(csm, reg=None, kappa=0.0, out=None, stream=None):
+    """The CG-SENSE gadget's preconditioner (gpuCgSenseGadget.cpp:165-176):
+    1 / sqrt(sum_c |csm_c|^2 + kappa reg) as complex64 of the image shape.
+    csm: (C, *image) complex64 device array; reg: None or float32 image."""
+    from .nfft import DeviceArray
+    shp = tuple(int(v) for v in csm.shape)
+    C, img = shp[0], shp[1:]
+    nimg = int(np.prod(img))
+    if out is None:
+        out = DeviceArray.zeros(img, np.complex64)
+    check(lib().gt_cgsense_precond_weights(_ptr(csm), nimg, C, _ptr(reg), float(kappa), _ptr(out), _stream(stream)))
+    return out

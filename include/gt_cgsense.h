@@ -10,10 +10,11 @@
  *
  *   E x    = NFFT_C2NC(dcw; csm .* x)              x: [N..., F], y: [M, F, C]
  *   E^H y  = sum_c conj(csm_c) .* NFFT_NC2C(dcw; y)
- *   CG     : x_0 = 0, r = p = w_E E^H d, then the reference recurrence with
- *            alpha = rq / <p, q> (complex), <a, b> = sum conj(a) b, the
- *            preconditioner applied twice, identity regularisation kappa and
- *            termination after the iteration where rq / rq_0 < tolerance.
+ *   CG     : x_0 = 0 (or a given x0), r = p = w_E E^H d, then the reference
+ *            recurrence with alpha = rq / <p, q> (complex), <a, b> = sum conj(a) b,
+ *            the preconditioner applied twice, identity (kappa) and image
+ *            (imageOperator) regularisers and termination after the iteration
+ *            where rq / rq_0 < tolerance.
  *
  * Arithmetic is complex FP32; the dot products and norms that set the step
  * sizes are reduced in FP64 on the device, in a fixed order (results repeat
@@ -48,6 +49,16 @@ gt_status gt_sense_mult_M(gt_sense_op* op, const void* in, void* out, int accumu
 gt_status gt_sense_mult_MH(gt_sense_op* op, const void* in, void* out, int accumulate, gt_stream_t stream);
 gt_status gt_sense_mult_MH_M(gt_sense_op* op, const void* in, void* out, int accumulate, gt_stream_t stream);
 
+/* An image regularisation operator in the CG sum (cgSolver::add_regularization_operator,
+ * cgSolver.h:376-384): weight * R^H R x with R = imageOperator's real image w
+ * (imageOperator::mult_MH_M, operators/imageOperator.h:49-66: x .* w .* w).
+ * image: [N..., F] floats, broadcast by modulus; NULL is the identity. */
+typedef struct gt_cg_reg {
+    double weight;        /* linearOperator::set_weight of the regulariser */
+    const float* image;
+    size_t image_len;
+} gt_cg_reg;
+
 typedef struct gt_cg_desc {
     int iterations;        /* cgSolver::set_max_iterations */
     double tc_tolerance;   /* relativeResidualTCB: stop after rq / rq_0 < tc_tolerance */
@@ -56,12 +67,21 @@ typedef struct gt_cg_desc {
     const void* precond;   /* NULL or complex64 diagonal weights, broadcast by modulus
                               (cgPreconditioner::apply) */
     size_t precond_len;
+    const gt_cg_reg* regs; /* NULL or n_regs image regularisers, summed after E^H E in
+                              list order (cgSolver::mult_MH_M, cgSolver.h:355-390) */
+    int n_regs;
+    const void* x0;        /* NULL or the initial guess [N..., F] complex64 (solver::set_x0;
+                              cgSolver::initialize, cgSolver.h:240-280): x = x0,
+                              r = rhs - A x0, p = P^2 r; rq_0 stays that of the
+                              right-hand side, as in the reference */
 } gt_cg_desc;
 
-/* cgSolver::solve(d): x = CG(E^H E + kappa I, w_E E^H d).  data: [M, F, C];
- * x: [N..., F] (written).  rq_history: NULL or iterations + 1 doubles receiving
- * rq_0 and rq after each iteration (0 after termination).  The iterations run
- * without host synchronisation. */
+/* cgSolver::solve(d): x = CG(A, w_E E^H d) with A = w_E E^H E + kappa I +
+ * sum_k weight_k diag(image_k)^2.  data: [M, F, C]; x: [N..., F] (written).
+ * iterations == 0 returns the right-hand side w_E E^H d, as
+ * cgSolver::solve_from_rhs does (cgSolver.h:124-129).  rq_history: NULL or
+ * iterations + 1 doubles receiving rq_0 and rq after each iteration (0 after
+ * termination).  The iterations run without host synchronisation. */
 gt_status gt_cgsense_solve(gt_sense_op* op, const void* data, void* x, const gt_cg_desc* desc,
                            double* rq_history, gt_stream_t stream);
 
@@ -75,7 +95,7 @@ gt_status gt_cgsense_solve_image(gt_sense_op* op, const void* rhs, void* x, cons
 
 /* Many independent reconstructions (uneven sizes, one operator each), set up
  * once and run any number of times (the operators, data and x buffers are
- * bound at creation).
+ * bound at creation; every problem needs its own operator, plan and x buffer).
  *   GT_CG_PLAIN:   one problem at a time, each iteration's dot products read
  *                  back to the host, as the reference's cgSolver does;
  *   GT_CG_BATCHED: every problem's whole solve (all iterations) captured into
@@ -85,6 +105,14 @@ gt_status gt_cgsense_solve_image(gt_sense_op* op, const void* rhs, void* x, cons
  * Both modes compute the same arithmetic (results agree bit for bit). */
 enum { GT_CG_PLAIN = 0, GT_CG_BATCHED = 1 };
 typedef struct gt_cg_batch gt_cg_batch;
+/* gt_cgsense_batch_create (GT_CG_BATCHED) times candidate graphs by running
+ * them: it writes every problem's x (whole solves), reads the data and runs
+ * one E^H E per operator to size its workspaces.  After creation the caller
+ * may change the contents of data (and x, which each run overwrites) but not
+ * the bound plans (no preprocess or set_tiling) nor the operators' sizes:
+ * gt_cgsense_batch_run then returns GT_ERR_INVALID_ARGUMENT without running
+ * anything, and the batch must be created again.  desc->x0 is not taken by a
+ * batch (GT_ERR_UNSUPPORTED). */
 gt_status gt_cgsense_batch_create(gt_cg_batch** batch, gt_sense_op* const* ops, const void* const* data,
                                   void* const* x, int nprob, const gt_cg_desc* desc, int mode, gt_stream_t stream);
 gt_status gt_cgsense_batch_run(gt_cg_batch* batch, gt_stream_t stream);
@@ -92,6 +120,24 @@ gt_status gt_cgsense_batch_destroy(gt_cg_batch* batch);
 /* Parallel branches of a batched graph, picked at creation by timing one
  * replay per candidate (2, 4, ..., 64, and one per problem); 0 for GT_CG_PLAIN. */
 gt_status gt_cgsense_batch_branches(const gt_cg_batch* batch, int* branches);
+
+/* cuImageOperator<T>::compute (operators/imageOperator.h:28-47 with the
+ * histogram offset of operators/gpu/cuImageOperator.h:30-63): the real
+ * regularisation image 1 / (|x| s + offset) of a complex image x (n elements),
+ * s = n / asum(x) (asum = sum |re| + |im|, the BLAS definition).  With
+ * offset_estimation the offset is (k + 1) max / 50000, k the number of
+ * 50000-bin histogram bins of |x| s / max up to and including the bin that
+ * brings the count to floor(0.01 n); otherwise *offset is used as given.  The
+ * offset used is written back to *offset.  out: n floats. */
+gt_status gt_image_operator_compute(const void* image, size_t n, int offset_estimation, float* offset, float* out,
+                                    gt_stream_t stream);
+
+/* The CG-SENSE preconditioner weights of gpuCgSenseGadget::process
+ * (gadgets/pmri/gpuCgSenseGadget.cpp:165-176): w = 1 / sqrt(sum_c |csm_c|^2 +
+ * kappa reg) over the nimg image elements (reg NULL: 0), written as complex64
+ * (imaginary part 0).  csm: [N..., C] complex64; reg: nimg floats. */
+gt_status gt_cgsense_precond_weights(const void* csm, size_t nimg, size_t ncoils, const float* reg, double kappa,
+                                     void* weights, gt_stream_t stream);
 
 #ifdef __cplusplus
 }

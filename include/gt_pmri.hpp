@@ -12,6 +12,7 @@
 //                                             cuNonCartesianSenseOperator<REAL, D>
 //   solvers/gpu/cuCgSolver.h, solvers/cgSolver.h:105-390
 //                                             cuCgSolver<T> for that operator
+//   operators/gpu/cuImageOperator.h:16-66     cuImageOperator<T> (image regulariser)
 //
 // Arrays are duck typed as in gt_nufft.hpp (get_data_ptr(), get_dimensions()
 // with x fastest, get_number_of_elements()); results come back as
@@ -376,10 +377,58 @@ private:
     gt_sense_op* op_ = nullptr;
 };
 
+// cuImageOperator<float_complext> (operators/imageOperator.h:14-96,
+// operators/gpu/cuImageOperator.h:16-66): a regulariser with the real image
+// 1 / (|x| n / asum(x) + offset) computed from a complex image x.  get()
+// returns the operator's own image array, as the reference's shared_ptr does:
+// scaling it in place (gpuCgSenseGadget.cpp:169-170 does) changes the
+// operator.  As a CG regulariser it adds weight * image^2 * x.
+template <class REAL>
+class cuImageOperator {
+public:
+    cuImageOperator() { detail::require_float<REAL>(); }
+    void set_weight(REAL w) { weight_ = w; }
+    REAL get_weight() const { return weight_; }
+    template <class A>
+    void compute(A* image, bool offset_estimation = true) {
+        if (!image) throw std::runtime_error("cuImageOperator::compute: illegal image");
+        const auto dims = detail::dims_of(*image);
+        if (!image_ || image_->get_dimensions() != dims) image_ = std::make_shared<cuNDArray<REAL>>(dims);
+        float off = offset_;
+        check(gt_image_operator_compute(image->get_data_ptr(), detail::elements(*image), offset_estimation ? 1 : 0,
+                                        &off, image_->get_data_ptr(), nullptr));
+        offset_ = off;
+    }
+    std::shared_ptr<cuNDArray<REAL>> get() { return image_; }
+    REAL get_offset() const { return offset_; }
+
+private:
+    REAL weight_ = REAL(1);
+    REAL offset_ = REAL(0);
+    std::shared_ptr<cuNDArray<REAL>> image_;
+};
+
+// The CG-SENSE gadget's preconditioner weights (gpuCgSenseGadget.cpp:165-176):
+// 1 / sqrt(sum_c |csm_c|^2 + kappa * reg) as complex, reg may be null.
+template <class CSM, class REG>
+std::shared_ptr<cuNDArray<std::complex<float>>> cg_sense_precond_weights(CSM* csm, REG* reg, float kappa) {
+    if (!csm) throw std::runtime_error("cg_sense_precond_weights: illegal coil maps");
+    auto dims = detail::dims_of(*csm);
+    const size_t C = dims.back();
+    dims.pop_back();
+    auto w = std::make_shared<cuNDArray<std::complex<float>>>(dims);
+    check(gt_cgsense_precond_weights(csm->get_data_ptr(), w->get_number_of_elements(), C,
+                                     reg ? reinterpret_cast<const float*>(reg->get_data_ptr()) : nullptr, kappa,
+                                     w->get_data_ptr(), nullptr));
+    return w;
+}
+
 // cuCgSolver for a cuNonCartesianSenseOperator (cgSolver.h:105-390): solve(d)
-// returns x = CG(w_E E^H E + kappa I, w_E E^H d), the iterations and dot
-// products on the device.  An identity regulariser of weight kappa stands for
-// add_regularization_operator(identity) with that weight.
+// returns x = CG(w_E E^H E + kappa I + sum_k w_k R_k^H R_k, w_E E^H d), the
+// iterations and dot products on the device.  An identity regulariser of
+// weight kappa stands for add_regularization_operator(identity) with that
+// weight; image regularisers are cuImageOperator objects (their current image
+// and weight are read at each solve).
 template <class REAL, unsigned int D>
 class cuCgSolver {
 public:
@@ -387,6 +436,14 @@ public:
     void set_max_iterations(unsigned int n) { iterations_ = n; }
     void set_tc_tolerance(REAL t) { tolerance_ = t; }
     void set_identity_regularization(REAL kappa) { kappa_ = kappa; }
+    // cgSolver::add_regularization_operator (an image operator)
+    void add_regularization_operator(std::shared_ptr<cuImageOperator<REAL>> R) { regs_.push_back(std::move(R)); }
+    // solver::set_x0: the initial guess [N..., F] (kept by reference)
+    template <class A>
+    void set_x0(std::shared_ptr<A> x0) {
+        x0_keep_ = x0;
+        x0_ = x0 ? static_cast<const void*>(x0->get_data_ptr()) : nullptr;
+    }
     // cgPreconditioner::set_weights: complex diagonal weights, kept by reference.
     template <class A>
     void set_preconditioner_weights(std::shared_ptr<A> w) {
@@ -409,6 +466,15 @@ public:
         d.kappa = kappa_;
         d.precond = precond_;
         d.precond_len = precond_len_;
+        std::vector<gt_cg_reg> regs;
+        for (auto& R : regs_) {
+            auto img = R->get();
+            if (!img) throw std::runtime_error("cgSolver::solve: regularisation image not computed");
+            regs.push_back(gt_cg_reg{double(R->get_weight()), img->get_data_ptr(), img->get_number_of_elements()});
+        }
+        d.regs = regs.empty() ? nullptr : regs.data();
+        d.n_regs = int(regs.size());
+        d.x0 = x0_;
         history_.assign(iterations_ + 1, 0.0);
         check(gt_cgsense_solve(E_->op(), data->get_data_ptr(), x->get_data_ptr(), &d, history_.data(), nullptr));
         check(gt_stream_synchronize(nullptr));
@@ -422,6 +488,9 @@ private:
     std::shared_ptr<void> precond_keep_;
     const void* precond_ = nullptr;
     size_t precond_len_ = 0;
+    std::vector<std::shared_ptr<cuImageOperator<REAL>>> regs_;
+    std::shared_ptr<void> x0_keep_;
+    const void* x0_ = nullptr;
     std::vector<double> history_;
 };
 

@@ -18,6 +18,12 @@
 //          comb.bin  [N, N]     complex64   combined coil image
 //          mh.bin    [N, N]     complex64   E^H of frame 0 (sqrt dcw)
 //          cg.bin    [N, N]     complex64   5 CG iterations on frame 0
+//          cgreg.bin [N, N]     complex64   5 CG iterations on frame 0 in the
+//                               order of gpuCgSenseGadget::process
+//                               (gadgets/pmri/gpuCgSenseGadget.cpp:158-190):
+//                               cuImageOperator from the combined image,
+//                               weight kappa 0.05, its image scaled by kappa in
+//                               place, preconditioner 1/sqrt(sum |csm|^2 + that)
 // and checks that invalid calls throw std::runtime_error.  Exit status 0 on success.
 #include <cmath>
 #include <complex>
@@ -158,6 +164,31 @@ int main(int argc, char** argv) {
         std::cout << "rq";
         for (double r : cg.get_residual_history()) std::cout << " " << r;
         std::cout << "\n";
+
+        // the CG-SENSE gadget's solve: regulariser from the combined image
+        const float kappa = 0.05f;
+        auto comb = buf.get_combined_coil_image();
+        auto R = std::make_shared<cuImageOperator<float>>();
+        R->set_weight(kappa);
+        R->compute(comb.get());
+        {
+            // R_diag = R->get(); *R_diag *= kappa: the operator's own image
+            auto Rd = R->get();
+            std::vector<float> h(Rd->get_number_of_elements());
+            Rd->download(h.data());
+            for (float& v : h) v *= kappa;
+            Rd->upload(h.data());
+        }
+        auto pw = cg_sense_precond_weights(csm.get(), R->get().get(), 1.0f);
+        cuCgSolver<float, 2> cgr;
+        cgr.set_encoding_operator(E);
+        cgr.add_regularization_operator(R);
+        cgr.set_preconditioner_weights(pw);
+        cgr.set_max_iterations(5);
+        cgr.set_tc_tolerance(1e-12f);
+        auto xr = cgr.solve(&d0);
+        dump(dir + "/cgreg.bin", *xr);
+        std::cout << "offset " << R->get_offset() << "\n";
 
         // error paths
         bool ok = true;

@@ -134,3 +134,12 @@ def test_cpp_pmri_pipeline(gpu, tmp_path):
     assert rel_l2(mh, op.mult_MH(samples[0][:, None])) <= 1e-5
     xo, _ = ocg.cg_solve(op, samples[0][:, None], 5, tc_tolerance=1e-12)
     assert rel_l2(x, xo) <= 1e-4
+    # gpuCgSenseGadget's order: R from the combined image, its image scaled by
+    # kappa in place (the reference's get() aliases it), precond 1/sqrt(sum |csm|^2 + kappa R)
+    kappa = 0.05
+    xr = load("cgreg.bin", np.complex64, (1, N, N))
+    rimg, _ = ocg.image_operator_compute(comb)
+    reg_img = (np.float32(kappa) * rimg).astype(np.float32)
+    w = ocg.precond_weights(csm, reg_img, 1.0)
+    xro, _ = ocg.cg_solve(op, samples[0][:, None], 5, tc_tolerance=1e-12, precond=w, regs=[(kappa, reg_img)])
+    assert rel_l2(xr, xro) <= 1e-4

@@ -96,7 +96,8 @@ def test_cgsense_matches_oracle(gpu, kappa, use_precond):
     w = None
     d_w = None
     if use_precond:
-        w = (1.0 / np.sqrt(np.sum(np.abs(P.csm) ** 2, axis=0) + 0.1)).astype(np.complex64)
+        # weights that vary over the image (a constant one leaves the iterates unchanged)
+        w = np.random.default_rng(7).uniform(0.4, 2.5, (1, 96, 96)).astype(np.complex64)
         d_w = dev(w, np.complex64)
     x = DeviceArray.zeros((1, 96, 96), np.complex64)
     hist = ga.cg_sense(P.op, P.d_data, x, iterations=10, tc_tolerance=1e-12, kappa=kappa, precond=d_w)
@@ -140,3 +141,78 @@ def test_batched_equals_plain_and_single(gpu):
         assert np.array_equal(xs.numpy(), xa)
         xo, _ = cg.cg_solve(p.oop, p.data, 10)
         assert rel_l2(xa, xo) <= 1e-4
+
+
+def test_batched_refuses_replay_after_plan_changes(gpu):
+    say("batched CG-SENSE after a bound plan is preprocessed again: run must refuse")
+    probs = [Problem(20, 64, 16, 3), Problem(21, 96, 24, 4)]
+    xs = [DeviceArray.zeros((1, p.N, p.N), np.complex64) for p in probs]
+    batched = ga.CgBatch([p.op for p in probs], [p.d_data for p in probs], xs, iterations=4, mode="batched")
+    batched.run()
+    probs[0].plan.synchronize()
+    # the same trajectory again: the tables are rebuilt in place, the graph holds stale launch sizes
+    probs[1].plan.preprocess(probs[1].d_traj)
+    with pytest.raises(ga.GTError) as e:
+        batched.run()
+    assert e.value.status == 1 and "create the batch again" in str(e.value)
+    # a new batch over the same operators works and agrees with single solves
+    again = ga.CgBatch([p.op for p in probs], [p.d_data for p in probs], xs, iterations=4, mode="batched")
+    again.run()
+    probs[0].plan.synchronize()
+    for p, x in zip(probs, xs):
+        xs1 = DeviceArray.zeros((1, p.N, p.N), np.complex64)
+        ga.cg_sense(p.op, p.d_data, xs1, iterations=4)
+        assert np.array_equal(xs1.numpy(), x.numpy())
+
+
+def test_image_operator_and_precond_weights_match_oracle(gpu):
+    say("cuImageOperator::compute and the gadget's preconditioner weights, 96^2, 6 coils")
+    rng = np.random.default_rng(30)
+    img = (rng.gamma(2.0, 1.0, (1, 96, 96)) * np.exp(2j * np.pi * rng.uniform(0, 1, (1, 96, 96)))).astype(np.complex64)
+    R = ga.sense.ImageOperator()
+    R.compute(dev(img, np.complex64))
+    ref, off = cg.image_operator_compute(img)
+    assert R.offset == pytest.approx(off, rel=1e-6)
+    assert rel_l2(R.image.numpy(), ref) <= 1e-6
+    # a given offset is used as is
+    R2 = ga.sense.ImageOperator()
+    R2.offset = 0.25
+    R2.compute(dev(img, np.complex64), offset_estimation=False)
+    ref2, _ = cg.image_operator_compute(img, offset_estimation=False, offset=0.25)
+    assert rel_l2(R2.image.numpy(), ref2) <= 1e-6
+    csm = smooth_csm(rng, 6, 96)
+    w = ga.sense.precond_weights(dev(csm, np.complex64), R.image, kappa=0.3)
+    assert rel_l2(w.numpy(), cg.precond_weights(csm, ref, 0.3)) <= 1e-6
+
+
+@pytest.mark.parametrize("with_x0", [False, True])
+def test_cgsense_regularised_preconditioned_matches_oracle(gpu, with_x0):
+    say(f"CG-SENSE with an image regulariser, gadget preconditioner, x0 {with_x0}: 96^2, 48 spokes, 6 coils")
+    P = Problem(3, 96, 48, 6)
+    rng = np.random.default_rng(31)
+    # the gadget's order: R from a reference image, then precond 1 / sqrt(sum |csm|^2 + kappa R)
+    ref_img = (P.x_true + 0.1 * (rng.standard_normal(P.x_true.shape) + 1j * rng.standard_normal(P.x_true.shape))
+               ).astype(np.complex64)
+    R = ga.sense.ImageOperator()
+    R.compute(dev(ref_img, np.complex64))
+    rimg, _ = cg.image_operator_compute(ref_img)
+    kappa = 0.1
+    w = ga.sense.precond_weights(P.d_csm, R.image, kappa=kappa)
+    w_ref = cg.precond_weights(P.csm, rimg, kappa)
+    x0 = (rng.standard_normal((1, 96, 96)) + 1j * rng.standard_normal((1, 96, 96))).astype(np.complex64)
+    x = dev(x0, np.complex64) if with_x0 else DeviceArray.zeros((1, 96, 96), np.complex64)
+    hist = ga.cg_sense(P.op, P.d_data, x, iterations=10, tc_tolerance=0.0, precond=w, regs=[(kappa, R.image)],
+                       x0=x if with_x0 else None)  # x is its own initial guess
+    xo, ho = cg.cg_solve(P.oop, P.data, 10, tc_tolerance=0.0, precond=w_ref, regs=[(kappa, rimg)],
+                         x0=x0 if with_x0 else None)
+    assert rel_l2(x.numpy(), xo) <= 1e-4
+    assert np.allclose(hist[: len(ho)], ho, rtol=1e-3)
+
+
+def test_cgsense_zero_iterations_returns_rhs(gpu):
+    say("CG-SENSE with zero iterations returns w_E E^H d, 64^2, 24 spokes, 3 coils")
+    P = Problem(4, 64, 24, 3)
+    x = DeviceArray.zeros((1, 64, 64), np.complex64)
+    ga.cg_sense(P.op, P.d_data, x, iterations=0, encoding_weight=0.5)
+    xo, _ = cg.cg_solve(P.oop, P.data, 0, encoding_weight=0.5)
+    assert rel_l2(x.numpy(), xo) <= 1e-5
