@@ -2058,8 +2058,11 @@ gt_status gt_device_count(int* count) {
     return guarded([&] {
         GT_REQUIRE(count, GT_ERR_INVALID_ARGUMENT, "null argument");
         int n = 0;
-        hipError_t e = hipGetDeviceCount(&n);
-        *count = (e == hipSuccess) ? n : 0;
+        *count = 0;
+        const hipError_t e = hipGetDeviceCount(&n);
+        if (e == hipErrorNoDevice) return;  // a host without GPUs: count 0
+        GT_HIP(e);
+        *count = n;
     });
 }
 gt_status gt_set_device(int device) { return guarded([&] { GT_HIP(hipSetDevice(device)); }); }

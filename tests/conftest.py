@@ -15,14 +15,14 @@ def pytest_configure(config):
 def _gpu_count():
     try:
         from gadgetron_amd import device_count
-        return device_count()
-    except Exception:
-        return 0
+        return device_count(), ""
+    except Exception as e:  # reported by the fixture
+        return 0, f"{type(e).__name__}: {e}"
 
 
 @pytest.fixture(scope="session")
 def gpu():
-    n = _gpu_count()
+    n, why = _gpu_count()
     if n == 0:
-        pytest.fail("no GPU visible to libgtnufft.so; gpu tests must run on an MI355X")
+        pytest.fail(f"no GPU visible to libgtnufft.so; gpu tests must run on an MI355X ({why or 'count 0'})")
     return n
