@@ -16,6 +16,7 @@ EXPORTED_SYMBOLS = [
     "gt_nufft_destroy",
     "gt_nufft_preprocess",
     "gt_nufft_compute",
+    "gt_nufft_compute_noalloc",
     "gt_nufft_mult_MH_M",
     "gt_nufft_convolve",
     "gt_nufft_fft",
@@ -158,6 +159,7 @@ def lib():
         "gt_nufft_destroy": [P],
         "gt_nufft_preprocess": [P, P, S, S, I, P],
         "gt_nufft_compute": [P, P, SP, I, P, SP, I, P, S, I, P],
+        "gt_nufft_compute_noalloc": [P, P, SP, I, P, SP, I, P, S, I, P],
         "gt_nufft_mult_MH_M": [P, P, P, SP, I, P, S, P],
         "gt_nufft_convolve": [P, P, SP, I, P, SP, I, I, I, P],
         "gt_nufft_fft": [P, P, SP, I, I, I, P],

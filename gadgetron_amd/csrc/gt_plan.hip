@@ -1516,14 +1516,17 @@ gt::FftGeom fft_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float sca
 // cropped rows, then row FFT + crop + deapodize (NC2C side).
 // csm (FORWARDS_C2NC only): coil maps [N..., C]; `in` is then one image per
 // frame, multiplied by each coil's map inside the row pass.
+// grid_ext / partials_ext: caller-provided grid (nb * G complex) and split-tile
+// partial sums; null: the plan's workspaces, grown here.
 void compute_fast2d_fused(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const float* dcw,
-                          size_t dcw_len, int mode, float scale, hipStream_t s, const float2* csm = nullptr) {
+                          size_t dcw_len, int mode, float scale, hipStream_t s, const float2* csm = nullptr,
+                          float2* grid_ext = nullptr, void* partials_ext = nullptr) {
     const size_t nb = sh.nb;
     const int Ox = int(p->Nos[0]), Oy = int(p->Nos[1]);
     GT_REQUIRE(nb * p->N[1] < (size_t(1) << 31) && nb * size_t((Ox + 15) / 16) < (size_t(1) << 31),
                GT_ERR_UNSUPPORTED, "too many batches for one launch");
-    p->grid.reserve(nb * p->G() * sizeof(float2));
-    float2* g = p->grid.as<float2>();
+    if (!grid_ext) p->grid.reserve(nb * p->G() * sizeof(float2));
+    float2* g = grid_ext ? grid_ext : p->grid.as<float2>();
     const gt::Layout L = internal_layout(p, nb);
     const float2* twx = p->twiddle[0].as<float2>();
     const float2* twy = p->twiddle[1].as<float2>();
@@ -1543,7 +1546,7 @@ void compute_fast2d_fused(gt_nufft_plan* p, const void* in, void* out, const Sha
     } else {
         const float dir = mode == GT_FORWARDS_NC2C ? -1.0f : 1.0f;
         const gt::FftGeom fg = fft_geom(p, nb, false, scale);
-        conv_nc2c(p, in, g, L, sh.C, 1, dcw, dcw_len, false, s);
+        conv_nc2c(p, in, g, L, sh.C, 1, dcw, dcw_len, false, s, partials_ext);
         {
             StageTimer timer(p, GT_STAGE_FFT, s);
             dispatch_fft_len<LaunchColsBandOut>(Oy, g, Ox, fg, twy, dir, s);
@@ -1662,6 +1665,83 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
 
 }  // namespace
 
+// ---- fixed device scratch of gt_nufft_compute_noalloc -----------------------
+// The oversampled grid and the split tiles' partial sums of one call live here
+// instead of in the plan's grow-only workspaces.  Each GPU holds its own copy
+// of the array from the library's first use on it until the process ends.  In
+// host code `scratch` names a host-side stand-in, not device memory: its
+// device address is looked up once per device (scratch_address) and only that
+// address is handed to kernels.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex scratch_mu;
+std::map<int, void*> scratch_ptrs;  // device -> address of that device's `scratch`
+
+// Address of `scratch` on `device`, which must be the current device.  The
+// first call on a device looks it up; later calls return the kept address.
+void* scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    auto it = scratch_ptrs.find(device);
+    if (it != scratch_ptrs.end()) return it->second;
+    void* ptr = nullptr;
+    GT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    GT_REQUIRE(ptr, GT_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+    scratch_ptrs[device] = ptr;
+    return ptr;
+}
+
+// Stage timing draws HIP events from a pool that may grow; switched off for
+// the duration of a no-allocation call.
+struct ProfilingPaused {
+    gt_nufft_plan* p;
+    bool was;
+    explicit ProfilingPaused(gt_nufft_plan* p_) : p(p_), was(p_->profiling) { p->profiling = false; }
+    ~ProfilingPaused() { p->profiling = was; }
+};
+
+// gt_nufft_compute on the fused 2D path with the grid and partial tiles in
+// `scratch`: nothing is allocated.  Everything is checked on the host first.
+void compute_noalloc(gt_nufft_plan* p, const void* in, const size_t* in_dims, int in_ndim, void* out,
+                     const size_t* out_dims, int out_ndim, const float* dcw, size_t dcw_len, int mode,
+                     hipStream_t s) {
+    GT_REQUIRE(in && out, GT_ERR_INVALID_ARGUMENT, "null input or output");
+    GT_REQUIRE(mode >= 0 && mode <= 3, GT_ERR_INVALID_ARGUMENT, "unknown NFFT_comp_mode");
+    GT_REQUIRE(!dcw || dcw_len > 0, GT_ERR_INVALID_ARGUMENT, "dcw given with zero length");
+    GT_REQUIRE(!p->dbl && p->fast2d && p->fft_fused && fft2d_strategy() == 2, GT_ERR_UNSUPPORTED,
+               "gt_nufft_compute_noalloc takes float 2D plans whose oversampled sizes are fused FFT lengths "
+               "(96, 128, 144, 160, 192, 240, 256, 288, 320, 384)");
+    GT_REQUIRE(p->conv_type == GT_CONV_STANDARD, GT_ERR_UNSUPPORTED,
+               "gt_nufft_compute_noalloc takes STANDARD convolution only");
+    const bool image_in = (mode == GT_FORWARDS_C2NC || mode == GT_BACKWARDS_C2NC);
+    Shape sh = image_in ? image_shape(p, in_dims, in_ndim, true) : image_shape(p, out_dims, out_ndim, true);
+    const size_t selems = image_in ? elements(out_dims, out_ndim) : elements(in_dims, in_ndim);
+    check_batches(p, sh, selems);
+    GT_REQUIRE(!sh.oversampled, GT_ERR_UNSUPPORTED,
+               "gt_nufft_compute_noalloc takes an image of the matrix size, not the oversampled one");
+    const size_t grid_bytes = sh.nb * p->G() * sizeof(float2);
+    const size_t part_off = (grid_bytes + 255) & ~size_t(255);
+    const size_t part_bytes =
+        image_in ? 0 : size_t(p->bn.n_partials) * sh.C * size_t(p->bn.geom.tile_cells) * sizeof(float2);
+    if (part_off + part_bytes > kScratchBytes) {
+        std::ostringstream os;
+        os << "gt_nufft_compute_noalloc needs " << part_off + part_bytes << " bytes of scratch (grid "
+           << grid_bytes << ", partial tiles " << part_bytes << "), more than its fixed " << kScratchBytes
+           << " bytes; use gt_nufft_compute";
+        throw gt_host::gt_error(GT_ERR_UNSUPPORTED, os.str());
+    }
+    set_device(p);
+    unsigned char* base = static_cast<unsigned char*>(scratch_address(p->device));
+    ProfilingPaused paused(p);
+    const float scale = centring_global_sign(p) / std::sqrt(float(p->G()));
+    compute_fast2d_fused(p, in, out, sh, dcw, dcw_len, mode, scale, s, nullptr,
+                         reinterpret_cast<float2*>(base), base + part_off);
+}
+
+}  // namespace
+
 bool gt_host::plan_is_double(const gt_nufft_plan* p) { return p->dbl; }
 uint64_t gt_host::plan_generation(const gt_nufft_plan* p) { return p->generation; }
 
@@ -1751,6 +1831,7 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         p->device = dev;
         set_device(p.get());
         GT_HIP(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev));
+        (void)scratch_address(dev);  // once per device, with the device current
         for (int i = 0; i < p->D; i++) {
             p->filt_host[i] = deapod_1d(p->Nos[i], p->radius, p->one_over_W, p->beta[i]);
             p->filt[i].reserve(p->Nos[i] * sizeof(float));
@@ -1866,6 +1947,15 @@ gt_status gt_nufft_compute(gt_nufft_plan* p, const void* in, const size_t* in_di
     return guarded([&] {
         GT_REQUIRE(p, GT_ERR_INVALID_ARGUMENT, "null plan");
         compute_impl(p, in, in_dims, in_ndim, out, out_dims, out_ndim, dcw, dcw_len, comp_mode, S(stream));
+    });
+}
+
+gt_status gt_nufft_compute_noalloc(gt_nufft_plan* p, const void* in, const size_t* in_dims, int in_ndim, void* out,
+                                   const size_t* out_dims, int out_ndim, const float* dcw, size_t dcw_len,
+                                   int comp_mode, gt_stream_t stream) {
+    return guarded([&] {
+        GT_REQUIRE(p, GT_ERR_INVALID_ARGUMENT, "null plan");
+        compute_noalloc(p, in, in_dims, in_ndim, out, out_dims, out_ndim, dcw, dcw_len, comp_mode, S(stream));
     });
 }
 

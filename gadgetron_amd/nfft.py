@@ -231,6 +231,16 @@ class NFFTPlan:
         check(lib().gt_nufft_compute(self._h, _ptr(inp), sizes(di), len(di), _ptr(out), sizes(do), len(do),
                                      _ptr(dcw), dl, int(mode), _stream(stream)))
 
+    def compute_noalloc(self, inp, out, dcw, mode, stream=None):
+        """compute() without any device allocation (gt_nufft_compute_noalloc):
+        the grid lives in the library's fixed 64 MiB per-GPU scratch array.
+        Float 2D STANDARD plans with fused FFT lengths only; calls that overlap
+        on one GPU race."""
+        di, do = _gt_dims(inp), _gt_dims(out)
+        dl = 0 if dcw is None else _numel(dcw)
+        check(lib().gt_nufft_compute_noalloc(self._h, _ptr(inp), sizes(di), len(di), _ptr(out), sizes(do),
+                                             len(do), _ptr(dcw), dl, int(mode), _stream(stream)))
+
     def mult_MH_M(self, inp, out, dcw=None, stream=None):
         d = _gt_dims(inp)
         dl = 0 if dcw is None else _numel(dcw)

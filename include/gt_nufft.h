@@ -97,6 +97,27 @@ gt_status gt_nufft_compute(gt_nufft_plan* plan, const void* in, const size_t* in
                            void* out, const size_t* out_dims, int out_ndim, const float* dcw,
                            size_t dcw_len, int comp_mode, gt_stream_t stream);
 
+/* gt_nufft_compute that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as gt_nufft_compute, but the oversampled grid
+ * and the split tiles' partial sums live in a fixed device array of the
+ * library, 64 MiB, of which each GPU holds its own copy from the library's
+ * first use on it until the process ends.  The call needs nb * prod(N_os) * 8
+ * bytes for the grid (nb = frames * coils) plus, for the NC2C modes, the
+ * partial tiles, together at most 64 MiB (the 256^2, 32-coil workload: 36 MiB
+ * of grid); a larger call returns GT_ERR_UNSUPPORTED before any device work.
+ * Taken: float 2D STANDARD plans whose oversampled sizes are fused FFT lengths
+ * (96, 128, 144, 160, 192, 240, 256, 288, 320, 384), image side of the matrix
+ * size; anything else returns GT_ERR_UNSUPPORTED.  Nothing in the array needs
+ * to start at zero (every call writes what it reads).  Every call on a GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (several
+ * streams or threads) race, and the caller must order them.  Preprocess, which
+ * allocates, must have run before. */
+gt_status gt_nufft_compute_noalloc(gt_nufft_plan* plan, const void* in, const size_t* in_dims, int in_ndim,
+                                   void* out, const size_t* out_dims, int out_ndim, const float* dcw,
+                                   size_t dcw_len, int comp_mode, gt_stream_t stream);
+
 /* NFFT_plan::mult_MH_M (NFFT.hpp:89-101): FORWARDS_C2NC then BACKWARDS_NC2C. */
 gt_status gt_nufft_mult_MH_M(gt_nufft_plan* plan, const void* in, void* out, const size_t* dims,
                              int ndim, const float* dcw, size_t dcw_len, gt_stream_t stream);
