@@ -8,12 +8,18 @@ ramp DCF.  One step = BACKWARDS_NC2C (samples -> 32 coil images) followed by
 FORWARDS_C2NC (images -> samples), DCF applied in both, inputs resident in HBM.
 
 value = M*C / step time, in Msamples*coil/s (one direction's work units per
-adjoint+forward pair).  The roofline object covers the dominant gridding stage
-(NC2C or C2NC, timed with HIP events on the compute stream during the timed
-steps) against its algorithmic bytes 8*C*(M+G) + 4*D*M + 4*M per launch
-(samples, grid, trajectory, DCF each touched once).  The CPU baseline is the
-oracle (the hoNFFT restatement) timed on this host's cores on the same
-workload.
+adjoint+forward pair).  The timed steps record no events except around the two
+gridding stages of every 10th step (an event waits for the work before it, so
+recording every stage of every step would slow the step by about a sixth).
+The roofline object covers the dominant gridding stage (NC2C or C2NC, its mean
+HIP-event time on the compute stream over those sampled steps) against its
+algorithmic bytes 8*C*(M+G) + 4*D*M + 4*M per launch (samples, grid,
+trajectory, DCF each touched once).  The CPU baseline is the oracle (the
+hoNFFT restatement) timed on this host's cores on the same workload.
+
+The default run also reports the plain 128^2 one-coil latency, the 3D
+kooshball workload, the differentiable workload (forward + backward through
+torch autograd) and the batched uneven CG-SENSE, each with its own figures.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c5]
 """
@@ -58,15 +64,39 @@ def grid_bytes(N, M, C, D):
     return 8 * C * (M + G) + 4 * D * M + 4 * M, G
 
 
+def host_cpu():
+    """CPU model and core counts of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
 def cpu_baseline(wl, samples, budget_s):
+    """The oracle on this host's cores: OMP_NUM_THREADS threads where the
+    environment sets it (the share of the host this process is given), else
+    every core this process may run on."""
     from oracle import pyoracle as po
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or host["usable"])
     po.set_num_threads(threads)
     N, traj, C = wl["N"], wl["traj"], wl["C"]
     M = traj.shape[1]
     D = len(N)
     p = po.Plan(N, 5.5, alpha=1.5, precision="float")
+    t0 = time.perf_counter()
     p.preprocess(traj)
+    prep_ms = (time.perf_counter() - t0) * 1e3
     img_shape = (C, 1) + tuple(reversed(N))
     times = []
     t_end = time.perf_counter() + budget_s
@@ -82,8 +112,14 @@ def cpu_baseline(wl, samples, budget_s):
         "unit": "Msamples·coil/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle (hoNFFT restatement, FP32, OpenMP) on the full workload, {len(times)} adjoint+forward "
-                  f"pairs, median {t * 1e3:.1f} ms per pair, preprocess excluded",
+        "sample": f"oracle (hoNFFT restatement, FP32, OpenMP over batches and samples, a textbook radix-2/mixed "
+                  f"recursive FFT in place of FFTW) on the full workload, {len(times)} adjoint+forward pairs, "
+                  f"median {t * 1e3:.1f} ms per pair, preprocess excluded",
+        "cpu_model": host["model"],
+        "host_nproc": host["nproc"],
+        "host_usable_cores": host["usable"],
+        "preprocess_ms": round(prep_ms, 1),
+        "ms_per_pair": round(t * 1e3, 2),
     }
 
 
@@ -292,22 +328,222 @@ def load_traffic(config):
         return json.load(f)
 
 
+GRID_STAGES = ("nc2c", "c2nc")
+
+
+class NufftRun:
+    """One workload's plan and device buffers, and the step BACKWARDS_NC2C
+    (samples -> coil images) then FORWARDS_C2NC (images -> samples), DCF both
+    ways, on the default stream."""
+
+    def __init__(self, ga, wl, max_items=0):
+        from gadgetron_amd import DeviceArray
+        from gadgetron_amd import trajectories as tj
+        self.ga, self.wl = ga, wl
+        N, traj, C = wl["N"], wl["traj"], wl["C"]
+        self.M = traj.shape[1]
+        rng = np.random.default_rng(wl["seed"])
+        self.rng = rng
+        self.samples = tj.complex_normal(rng, (C, 1, self.M))
+        self.plan = ga.NFFTPlan(N, None, W=5.5, os_factor=1.5)
+        if max_items:
+            self.plan.set_tiling(None, max_items)
+        self.d_traj = DeviceArray.from_numpy(traj)
+        sync = ga.lib().gt_device_synchronize
+        sync()
+        t0 = time.perf_counter()
+        self.plan.preprocess(self.d_traj)
+        sync()
+        self.prep_ms = (time.perf_counter() - t0) * 1e3
+        self.d_s = DeviceArray.from_numpy(self.samples)
+        self.d_w = DeviceArray.from_numpy(wl["dcf"])
+        self.img_shape = (C, 1) + tuple(reversed(N))
+        self.d_img = DeviceArray.zeros(self.img_shape, np.complex64)
+        self.d_out = DeviceArray.zeros((C, 1, self.M), np.complex64)
+
+    def step(self):
+        from gadgetron_amd import NFFT_comp_mode as CM
+        self.plan.compute(self.d_s, self.d_img, self.d_w, CM.BACKWARDS_NC2C)
+        self.plan.compute(self.d_img, self.d_out, self.d_w, CM.FORWARDS_C2NC)
+
+    def timed(self, steps, warmup, sample_every=10):
+        """steps timed steps between device synchronisations; HIP events on
+        the compute stream around the two gridding stages of every
+        sample_every-th step.  Returns (ms per step, {stage: mean ms}, launches
+        sampled per stage)."""
+        sync = self.ga.lib().gt_device_synchronize
+        for _ in range(warmup):
+            self.step()
+        sync()
+        self.plan.profile_read()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            rec = i % sample_every == 0
+            if rec:
+                self.plan.profile_stages(GRID_STAGES)
+            self.step()
+            if rec:
+                self.plan.profile_stages(())
+        sync()
+        t1 = time.perf_counter()
+        st = self.plan.profile_read()
+        mean = {k: st[k][0] / max(st[k][1], 1) for k in GRID_STAGES}
+        return (t1 - t0) * 1e3 / steps, mean, st["nc2c"][1]
+
+    def breakdown(self, steps=10):
+        """Every stage recorded on every step (each event waits for the work
+        before it: these add up to more than an unrecorded step)."""
+        self.plan.profile_read()
+        self.plan.profile(True)
+        for _ in range(steps):
+            self.step()
+        self.plan.profile(False)
+        st = self.plan.profile_read()
+        return {k: round(v[0] / steps, 5) for k, v in st.items()}
+
+
+def roofline_of(wl, M, stage_ms, traffic=None):
+    N, C = wl["N"], wl["C"]
+    D = len(N)
+    B, _ = grid_bytes(N, M, C, D)
+    kernels = gridding_kernels(N)
+    flops = gridding_flops(N, M, C)
+    dom = "nc2c" if stage_ms["nc2c"] >= stage_ms["c2nc"] else "c2nc"
+    achieved = B / (stage_ms[dom] * 1e-3) / 1e9
+    out = {
+        "bound": "hbm",
+        "kernel": kernels[dom],
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None if traffic is None else traffic.get(dom, {}).get("bytes_per_launch"),
+        "algorithmic_bytes_per_launch": B,
+        "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        "stage_frac": {k: round(B / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for k, v in stage_ms.items()},
+        "stage_kernels": kernels,
+        # the FP32 vector side of the same stages (the 3D window makes gridding
+        # compute-heavy: 665.5 flop per sample and coil against 121 in 2D)
+        "stage_valu_tflops": {**{k: round(flops / (v * 1e-3) / 1e12, 2) for k, v in stage_ms.items()},
+                              "peak": FP32_PEAK_TFLOPS},
+    }
+    if traffic is not None:
+        out["traffic_source"] = traffic.get("measured", "profiles/traffic json")
+        out["traffic_ratio"] = {k: round(traffic[k]["bytes_per_launch"] / B, 3) for k in GRID_STAGES if k in traffic}
+    return out
+
+
+def secondary(ga, name, steps, warmup, sample_every, with_roofline):
+    """A non-headline workload's figures: step time, stage times, NDFT error."""
+    wl = workload(name)
+    run = NufftRun(ga, wl)
+    ms, stage_ms, nsamp = run.timed(steps, warmup, sample_every)
+    out = run.d_out.numpy()
+    if not np.all(np.isfinite(out)):
+        raise SystemExit(f"bench.py: non-finite output ({name})")
+    res = {
+        "workload": wl["name"],
+        "ms_per_step": round(ms, 5),
+        "us_per_step": round(ms * 1e3, 2),
+        "Msamples_coil_per_s": round(run.M * wl["C"] / (ms * 1e-3) / 1e6, 1),
+        "preprocess_ms": round(run.prep_ms, 2),
+        "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        "stages_ms_per_step_all_recorded": run.breakdown(min(steps, 10)),
+        "ndft_error": ndft_error(run.d_img.numpy(), out, wl, run.rng),
+    }
+    if with_roofline:
+        res["roofline"] = roofline_of(wl, run.M, stage_ms, load_traffic(name))
+    del run
+    return res
+
+
+def differentiable(iters=20):
+    """The differentiable workload: loss = ||A(x; k) - y||^2 at 256^2, 16 coils,
+    128 golden-angle spokes x 512 (seed 4), y from the trajectory perturbed by
+    N(0, 1e-3); one iteration = forward NUFFT + backward (image gradient by the
+    adjoint, trajectory gradient by the derivative-of-Kaiser-Bessel gather)."""
+    import torch
+    from gadgetron_amd import autograd as ag
+    from gadgetron_amd import trajectories as tj
+    wl = workload("c5")
+    traj, C, N = wl["traj"], wl["C"], wl["N"]
+    rng = np.random.default_rng(wl["seed"])
+    pert = np.clip(traj + rng.normal(0, 1e-3, traj.shape), -0.5, 0.5).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    x = torch.tensor(tj.complex_normal(rng, (C, 1, N[1], N[0])), device=dev, requires_grad=True)
+    k = torch.tensor(traj, device=dev, requires_grad=True)
+    with torch.no_grad():
+        y = ag.nufft(x.detach(), torch.tensor(pert, device=dev), N)
+
+    def it(kk):
+        x.grad = None
+        kk.grad = None
+        loss = (ag.nufft(x, kk, N) - y).abs().pow(2).sum()
+        loss.backward()
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    kd = k.detach()  # the same trajectory without a gradient
+    both = timed(lambda: it(k), iters)
+    x_only = timed(lambda: it(kd), iters)
+    fresh = timed(lambda: it(k.detach().clone().requires_grad_()), 5)
+    M = traj.shape[1]
+    return {
+        "workload": wl["name"] + ", loss ||A(x;k) - y||^2",
+        "fwd_bwd_ms": round(both, 3),
+        "fwd_bwd_Msamples_coil_per_s": round(M * C / (both * 1e-3) / 1e6, 1),
+        "fwd_bwd_image_grad_only_ms": round(x_only, 3),
+        "fwd_bwd_new_trajectory_ms": round(fresh, 3),
+        "note": "fwd_bwd: image and trajectory gradients, trajectory unchanged between iterations (its plan "
+                "is reused); image_grad_only: the trajectory without a gradient; new_trajectory: a new "
+                "trajectory tensor per iteration, so each pays preprocess; gradient accuracy is tested in "
+                "tests/test_gpu_autograd.py",
+    }
+
+
+def differentiable_child():
+    """differentiable() in a child process.  torch brings its own HIP runtime
+    with the same soname as the one libgtnufft.so links, and a process keeps
+    whichever loads first; this process loaded the library's, so torch runs in
+    a process of its own that imports it first."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--only-c5"], capture_output=True, text=True,
+                       timeout=600)
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"child exited {r.returncode}: {r.stderr[-400:]}"}
+    return json.loads(lines[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true", help="skip the batched CG-SENSE comparison")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the atomic NC2C, density compensation and coil-map timings (profiling runs)")
+                    help="skip the other workloads and side timings (profiling runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run only a device copy of this many MiB (counter calibration)")
     ap.add_argument("--max-items", type=int, default=0,
                     help="entries per gridding work item (gt_nufft_set_tiling); 0 keeps the default")
+    ap.add_argument("--only-c5", action="store_true",
+                    help="print only the differentiable workload's figures (run as a child process)")
     args = ap.parse_args()
+    if args.only_c5:
+        import torch  # noqa: F401  (torch's HIP runtime first; see differentiable_child)
+        print(json.dumps(differentiable()), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the NUFFT shards by coil "
               "with no exchange between GPUs).  Run with --gpus 1.", file=sys.stderr)
@@ -317,97 +553,47 @@ def main():
         return
 
     import gadgetron_amd as ga
-    from gadgetron_amd import DeviceArray, NFFT_comp_mode as CM
-    from gadgetron_amd import trajectories as tj
 
     if ga.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
     wl = workload(args.config)
-    N, traj, C = wl["N"], wl["traj"], wl["C"]
+    N, C = wl["N"], wl["C"]
     D = len(N)
-    M = traj.shape[1]
-    rng = np.random.default_rng(wl["seed"])
-    samples = tj.complex_normal(rng, (C, 1, M))
-
-    plan = ga.NFFTPlan(N, None, W=5.5, os_factor=1.5)
-    if args.max_items:
-        plan.set_tiling(None, args.max_items)
-    d_traj = DeviceArray.from_numpy(traj)
-    t0 = time.perf_counter()
-    plan.preprocess(d_traj)
-    plan.synchronize()
-    prep_ms = (time.perf_counter() - t0) * 1e3
-    d_s = DeviceArray.from_numpy(samples)
-    d_w = DeviceArray.from_numpy(wl["dcf"])
-    img_shape = (C, 1) + tuple(reversed(N))
-    d_img = DeviceArray.zeros(img_shape, np.complex64)
-    d_out = DeviceArray.zeros((C, 1, M), np.complex64)
-
-    def step():
-        plan.compute(d_s, d_img, d_w, CM.BACKWARDS_NC2C)
-        plan.compute(d_img, d_out, d_w, CM.FORWARDS_C2NC)
-
-    for _ in range(args.warmup):
-        step()
-    plan.synchronize()
-    plan.profile_read()
-    plan.profile(True)
-    plan.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    plan.synchronize()
-    t1 = time.perf_counter()
-    plan.profile(False)
-    stages = plan.profile_read()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    run = NufftRun(ga, wl, args.max_items)
+    M = run.M
+    # 3D steps take milliseconds: an event pair per gridding stage of every step costs ~0.1 %
+    every = 1 if D == 3 else 10
+    ms_per_step, stage_ms, sampled = run.timed(args.steps, args.warmup, every)
     value = M * C / (ms_per_step * 1e-3) / 1e6
+    roofline = roofline_of(wl, M, stage_ms, load_traffic(args.config))
+    roofline["stage_launches_timed"] = sampled
+    stages_all = run.breakdown(10)
 
-    B, G = grid_bytes(N, M, C, D)
-    nc2c_ms = stages["nc2c"][0] / max(stages["nc2c"][1], 1)
-    c2nc_ms = stages["c2nc"][0] / max(stages["c2nc"][1], 1)
-    kernels = gridding_kernels(N)
-    flops = gridding_flops(N, M, C)
-    # the dominant gridding stage is the roofline's kernel; both are reported
-    dom = "nc2c" if nc2c_ms >= c2nc_ms else "c2nc"
-    dom_ms = max(nc2c_ms, c2nc_ms)
-    achieved = B / (dom_ms * 1e-3) / 1e9
-    traffic = load_traffic(args.config)
-    roofline = {
-        "bound": "hbm",
-        "kernel": kernels[dom],
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None if traffic is None else traffic.get(dom, {}).get("bytes_per_launch"),
-        "algorithmic_bytes_per_launch": B,
-        "stage_ms": {"nc2c": round(nc2c_ms, 5), "c2nc": round(c2nc_ms, 5)},
-        "stage_frac": {"nc2c": round(B / (nc2c_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                       "c2nc": round(B / (c2nc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "stage_kernels": kernels,
-        # the FP32 vector side of the same stages (the 3D window makes gridding
-        # compute-heavy: 665.5 flop per sample and coil against 121 in 2D)
-        "stage_valu_tflops": {"nc2c": round(flops / (nc2c_ms * 1e-3) / 1e12, 2),
-                              "c2nc": round(flops / (c2nc_ms * 1e-3) / 1e12, 2),
-                              "peak": FP32_PEAK_TFLOPS},
-    }
-
-    # parity of this run's output on one coil against the oracle is covered by
-    # tests/; here a cheap sanity check keeps a broken build from printing a number
-    out = d_out.numpy()
+    out = run.d_out.numpy()
     if not np.all(np.isfinite(out)):
         raise SystemExit("bench.py: non-finite output")
-    ndft_err = ndft_error(d_img.numpy(), out, wl, rng)
+    ndft_err = ndft_error(run.d_img.numpy(), out, wl, run.rng)
 
     extras = not args.no_extras
-    atomic = atomic_comparison(ga, N, d_traj, d_s, d_w, img_shape) if D == 2 and extras else None
-    density = density_timing(ga, N, d_traj, 512, 400) if args.config == "c2" and extras else None
-    coil_maps = coil_map_timing(ga, N, d_traj, d_s, d_w, C) if args.config == "c2" and extras else None
+    atomic = atomic_comparison(ga, N, run.d_traj, run.d_s, run.d_w, run.img_shape) if D == 2 and extras else None
+    density = density_timing(ga, N, run.d_traj, 512, 400) if args.config == "c2" and extras else None
+    coil_maps = coil_map_timing(ga, N, run.d_traj, run.d_s, run.d_w, C) if args.config == "c2" and extras else None
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(wl, samples, args.cpu_seconds)
+        cpu = cpu_baseline(wl, run.samples, args.cpu_seconds)
+    prep_ms = run.prep_ms
+    del run
+
+    others = {}
+    if args.config == "c2" and extras:
+        for name, fn in (("c1_latency", lambda: secondary(ga, "c1", 2000, 500, 10, False)),
+                         ("c3", lambda: secondary(ga, "c3", 10, 2, 1, True)),
+                         ("c5_differentiable", differentiable_child)):
+            try:
+                others[name] = fn()
+            except Exception as e:  # reported, never hides the headline number
+                others[name] = {"error": f"{type(e).__name__}: {e}"}
 
     cgb = None
     if not args.no_cg:
@@ -438,8 +624,9 @@ def main():
         "density_compensation": density,
         "coil_maps": coil_maps,
         "ndft_error": ndft_err,
-        "stages_ms_per_step": {k: round(v[0] / args.steps, 5) for k, v in stages.items()},
+        "stages_ms_per_step_all_recorded": stages_all,
         "preprocess_ms": round(prep_ms, 2),
+        **others,
     }
     print(json.dumps(line), flush=True)
 

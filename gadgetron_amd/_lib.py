@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = [
     "gt_nufft_get_num_samples_frames",
     "gt_nufft_set_tiling",
     "gt_nufft_profile",
+    "gt_nufft_profile_stages",
     "gt_nufft_profile_read",
     "gt_last_error",
     "gt_device_count",
@@ -171,6 +172,7 @@ def lib():
         "gt_nufft_get_num_samples_frames": [P, SP, SP],
         "gt_nufft_set_tiling": [P, ctypes.POINTER(I), I],
         "gt_nufft_profile": [P, I],
+        "gt_nufft_profile_stages": [P, U],
         "gt_nufft_profile_read": [P, ctypes.POINTER(D), ctypes.POINTER(I), I],
         "gt_last_error": [],
         "gt_device_count": [ctypes.POINTER(I)],

@@ -184,8 +184,9 @@ struct gt_nufft_plan {
 
     std::map<std::pair<int, size_t>, FFTPlan> ffts;
 
-    // stage timing (gt_nufft_profile)
+    // stage timing (gt_nufft_profile, gt_nufft_profile_stages)
     bool profiling = false;
+    unsigned stage_mask = 0;  // stages that record events while profiling
     struct Record {
         int stage;
         hipEvent_t a, b;
@@ -252,6 +253,10 @@ struct gt_nufft_plan {
 
 namespace {
 
+// Events gt_nufft_profile keeps ready (8 per adjoint+forward pair of the 2D
+// path: 128 pairs between reads before the pool has to grow).
+constexpr size_t kEventPoolSize = 1024;
+
 // Records a start/stop event pair around one stage when the plan profiles.
 struct StageTimer {
     gt_nufft_plan* p;
@@ -259,7 +264,7 @@ struct StageTimer {
     hipStream_t s;
     hipEvent_t a = nullptr;
     StageTimer(gt_nufft_plan* p_, int stage_, hipStream_t s_) : p(p_), stage(stage_), s(s_) {
-        if (p->profiling) {
+        if (p->profiling && ((p->stage_mask >> stage) & 1u)) {
             a = p->get_event();
             if (a && hipEventRecord(a, s) != hipSuccess) {
                 p->event_pool.push_back(a);
@@ -2113,9 +2118,24 @@ gt_status gt_nufft_get_num_samples_frames(const gt_nufft_plan* p, size_t* M, siz
 }
 
 gt_status gt_nufft_profile(gt_nufft_plan* p, int enable) {
+    return gt_nufft_profile_stages(p, enable ? (1u << GT_NUM_STAGES) - 1u : 0u);
+}
+
+gt_status gt_nufft_profile_stages(gt_nufft_plan* p, unsigned stage_mask) {
     return guarded([&] {
         GT_REQUIRE(p, GT_ERR_INVALID_ARGUMENT, "null plan");
-        p->profiling = enable != 0;
+        p->stage_mask = stage_mask & ((1u << GT_NUM_STAGES) - 1u);
+        p->profiling = p->stage_mask != 0;
+        if (p->profiling) {
+            // events come from the pool while recording: fill it now, so that a
+            // timed loop records events without creating any
+            set_device(p);
+            while (p->event_pool.size() < kEventPoolSize) {
+                hipEvent_t e = nullptr;
+                GT_HIP(hipEventCreate(&e));
+                p->event_pool.push_back(e);
+            }
+        }
     });
 }
 

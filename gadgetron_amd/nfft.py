@@ -207,6 +207,13 @@ class NFFTPlan:
         """Record HIP events around each stage (see gt_nufft_profile)."""
         check(lib().gt_nufft_profile(self._h, int(bool(enable))))
 
+    def profile_stages(self, stages):
+        """Record HIP events around the named stages only (empty: stop)."""
+        mask = 0
+        for st in stages:
+            mask |= 1 << self.STAGES.index(st)
+        check(lib().gt_nufft_profile_stages(self._h, mask))
+
     def profile_read(self):
         """{stage: (total_ms, launches)} since the last read."""
         n = len(self.STAGES)

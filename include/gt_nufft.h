@@ -161,11 +161,18 @@ gt_status gt_nufft_set_tiling(gt_nufft_plan* plan, const int* tile, int max_samp
  * stage of compute/convolve/fft (the role of GPUTimer, core/gpu/GPUTimer.h:16-67).
  * Stages: 0 NC2C gridding (incl. the split-tile reduction), 1 C2NC gridding,
  * 2 FFT, 3 pad+deapodize, 4 crop+deapodize.  enable=0 stops recording;
- * gt_nufft_profile_read synchronizes, returns total milliseconds and launch
- * counts per stage (n entries) and clears the record. */
+ * enabling creates a pool of 1024 events up front, so recording creates none
+ * until that many are outstanding.  gt_nufft_profile_read synchronizes,
+ * returns total milliseconds and launch counts per stage (n entries) and
+ * clears the record. */
 enum { GT_STAGE_NC2C = 0, GT_STAGE_C2NC = 1, GT_STAGE_FFT = 2, GT_STAGE_PAD = 3, GT_STAGE_CROP = 4,
        GT_NUM_STAGES = 5 };
 gt_status gt_nufft_profile(gt_nufft_plan* plan, int enable);
+/* As gt_nufft_profile, recording only the stages whose bit (1 << stage) is
+ * set in stage_mask (0 stops recording).  Every recorded event waits for the
+ * work before it, so a timed loop that records only the stages it needs keeps
+ * the others at their unrecorded speed. */
+gt_status gt_nufft_profile_stages(gt_nufft_plan* plan, unsigned stage_mask);
 gt_status gt_nufft_profile_read(gt_nufft_plan* plan, double* total_ms, int* counts, int n);
 
 /* Thread-local message of the last failing call on this thread. */
