@@ -132,7 +132,7 @@ __device__ __forceinline__ float2 precond2(const CgParams& P, long long i, float
 // Initialisation: r = rhs_scale * mhd; p = r (or P^2 r); x = 0 (x_mode 0),
 // x = r (1: what a solve of zero iterations returns) or x untouched (2: an
 // initial guess follows, which may be x itself); partial sums of <r, p>.
-__global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __restrict__ mhd, float2* __restrict__ x,
+__device__ __forceinline__ void cg_init_run(const float2* __restrict__ mhd, float2* __restrict__ x,
                                                               float2* __restrict__ r, float2* __restrict__ p,
                                                               CgParams P, float rhs_scale, int x_mode,
                                                               double2* __restrict__ partial) {
@@ -151,7 +151,14 @@ __global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __re
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kRedThreads) void cg_init_finish_kernel(const double2* __restrict__ partial, int nb,
+__global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __restrict__ mhd, float2* __restrict__ x,
+                                                              float2* __restrict__ r, float2* __restrict__ p,
+                                                              CgParams P, float rhs_scale, int x_mode,
+                                                              double2* __restrict__ partial) {
+    cg_init_run(mhd, x, r, p, P, rhs_scale, x_mode, partial);
+}
+
+__device__ __forceinline__ void cg_init_finish_run(const double2* __restrict__ partial, int nb,
                                                                      CgState* st, double* hist) {
     __shared__ double2 sh[kRedThreads];
     double2 acc = make_double2(0.0, 0.0);
@@ -171,9 +178,14 @@ __global__ __launch_bounds__(kRedThreads) void cg_init_finish_kernel(const doubl
     }
 }
 
+__global__ __launch_bounds__(kRedThreads) void cg_init_finish_kernel(const double2* __restrict__ partial, int nb,
+                                                                     CgState* st, double* hist) {
+    cg_init_finish_run(partial, nb, st, hist);
+}
+
 // q = w_E sum_c conj(csm_c) img_c + kappa p: the coil sum of E^H E p (in the
 // order of csm_conj_sum_kernel) fused with the CG step; partial sums of <p, q>.
-__global__ __launch_bounds__(kRedThreads) void cg_q_csm_kernel(const float2* __restrict__ img,
+__device__ __forceinline__ void cg_q_csm_run(const float2* __restrict__ img,
                                                                const float2* __restrict__ csm, long long nimg, int F,
                                                                int C, const float2* __restrict__ p,
                                                                float2* __restrict__ q, CgParams P,
@@ -198,6 +210,14 @@ __global__ __launch_bounds__(kRedThreads) void cg_q_csm_kernel(const float2* __r
     }
     const double2 s = block_sum(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_q_csm_kernel(const float2* __restrict__ img,
+                                                               const float2* __restrict__ csm, long long nimg, int F,
+                                                               int C, const float2* __restrict__ p,
+                                                               float2* __restrict__ q, CgParams P,
+                                                               double2* __restrict__ partial) {
+    cg_q_csm_run(img, csm, nimg, F, C, p, q, P, partial);
 }
 
 // Initial guess (cgSolver::initialize with x0, cgSolver.h:254-277): x = x0,
@@ -341,7 +361,7 @@ __device__ __forceinline__ double2 sum_partials(const double2* __restrict__ part
     return s;
 }
 
-__global__ __launch_bounds__(kRedThreads) void cg_xr_step_kernel(float2* __restrict__ x, float2* __restrict__ r,
+__device__ __forceinline__ void cg_xr_step_run(float2* __restrict__ x, float2* __restrict__ r,
                                                                  const float2* __restrict__ p, float2* __restrict__ q,
                                                                  CgParams P, const CgState* __restrict__ st,
                                                                  const double2* __restrict__ pq, int nb,
@@ -375,9 +395,17 @@ __global__ __launch_bounds__(kRedThreads) void cg_xr_step_kernel(float2* __restr
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+__global__ __launch_bounds__(kRedThreads) void cg_xr_step_kernel(float2* __restrict__ x, float2* __restrict__ r,
+                                                                 const float2* __restrict__ p, float2* __restrict__ q,
+                                                                 CgParams P, const CgState* __restrict__ st,
+                                                                 const double2* __restrict__ pq, int nb,
+                                                                 double2* __restrict__ partial) {
+    cg_xr_step_run(x, r, p, q, P, st, pq, nb, partial);
+}
+
 // p = z + beta p with beta from the <r, z> partials; block 0 writes the state
 // of the next iteration (cg_beta_kernel).
-__global__ __launch_bounds__(kRedThreads) void cg_p_step_kernel(float2* __restrict__ p, const float2* __restrict__ z,
+__device__ __forceinline__ void cg_p_step_run(float2* __restrict__ p, const float2* __restrict__ z,
                                                                 long long n, const CgState* __restrict__ st,
                                                                 CgState* __restrict__ next,
                                                                 const double2* __restrict__ rz, int nb,
@@ -405,6 +433,14 @@ __global__ __launch_bounds__(kRedThreads) void cg_p_step_kernel(float2* __restri
     }
 }
 
+__global__ __launch_bounds__(kRedThreads) void cg_p_step_kernel(float2* __restrict__ p, const float2* __restrict__ z,
+                                                                long long n, const CgState* __restrict__ st,
+                                                                CgState* __restrict__ next,
+                                                                const double2* __restrict__ rz, int nb,
+                                                                double* __restrict__ hist, int it, double tol) {
+    cg_p_step_run(p, z, n, st, next, rz, nb, hist, it, tol);
+}
+
 __global__ void cg_p_kernel(float2* __restrict__ p, const float2* __restrict__ z, long long n,
                             const CgState* __restrict__ st) {
     const float b = (float)st->beta;
@@ -412,6 +448,72 @@ __global__ void cg_p_kernel(float2* __restrict__ p, const float2* __restrict__ z
         const float2 pv = p[i], zv = z[i];
         p[i] = make_float2(b * pv.x + zv.x, b * pv.y + zv.y);
     }
+}
+
+// ---------------------------------------------------------------------------
+// The fused batch: one launch per step for every problem, grid (blocks,
+// problems), blockIdx.y selecting the problem.  Each problem's step runs the
+// same body as the single-problem kernel over its own vectors, partial sums
+// and state.
+// ---------------------------------------------------------------------------
+constexpr int kFusedRedBlocks = 64;  // partial sums per reduction and problem
+
+struct CgProb {
+    float2* x;
+    float2* r;
+    float2* p;
+    float2* q;
+    float2* mhd;
+    const float2* coil;  // coil images of E^H (before the coil sum), [N..., F, C]
+    const float2* csm;
+    long long nimg;
+    int F, C;
+    CgParams P;
+    CgState* st;    // iterations + 1 states
+    double2* part;  // 2 kFusedRedBlocks partial sums
+};
+
+// mhd = sum_c conj(csm_c) coil_c (the coil sum of E^H d), as csm_conj_sum_kernel.
+__global__ __launch_bounds__(kRedThreads) void cg_coil_sum_batch_kernel(const CgProb* __restrict__ probs) {
+    const CgProb d = probs[blockIdx.y];
+    for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < d.P.n;
+         i += (long long)gridDim.x * kRedThreads) {
+        const long long f = i / d.nimg, n = i - f * d.nimg;
+        float2 a = make_float2(0.0f, 0.0f);
+        for (int c = 0; c < d.C; c++) {
+            const float2 s = d.csm[n + d.nimg * c];
+            const float2 v = d.coil[n + d.nimg * (f + (long long)d.F * c)];
+            a.x += s.x * v.x + s.y * v.y;
+            a.y += s.x * v.y - s.y * v.x;
+        }
+        d.mhd[i] = a;
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_init_batch_kernel(const CgProb* __restrict__ probs) {
+    const CgProb d = probs[blockIdx.y];
+    cg_init_run(d.mhd, d.x, d.r, d.p, d.P, d.P.we_re, 0, d.part);
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_init_finish_batch_kernel(const CgProb* __restrict__ probs) {
+    const CgProb d = probs[blockIdx.y];
+    cg_init_finish_run(d.part, kFusedRedBlocks, d.st, nullptr);
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_q_csm_batch_kernel(const CgProb* __restrict__ probs) {
+    const CgProb d = probs[blockIdx.y];
+    cg_q_csm_run(d.coil, d.csm, d.nimg, d.F, d.C, d.p, d.q, d.P, d.part);
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_xr_step_batch_kernel(const CgProb* __restrict__ probs, int it) {
+    const CgProb d = probs[blockIdx.y];
+    cg_xr_step_run(d.x, d.r, d.p, d.q, d.P, d.st + it, d.part, kFusedRedBlocks, d.part + kFusedRedBlocks);
+}
+
+__global__ __launch_bounds__(kRedThreads) void cg_p_step_batch_kernel(const CgProb* __restrict__ probs, int it) {
+    const CgProb d = probs[blockIdx.y];
+    cg_p_step_run(d.p, d.r, d.P.n, d.st + it, d.st + it + 1, d.part + kFusedRedBlocks, kFusedRedBlocks, nullptr, it,
+                  d.P.tol);
 }
 
 // ---------------------------------------------------------------------------
@@ -802,11 +904,18 @@ struct gt_cg_batch {
     std::vector<std::pair<uint64_t, uint64_t>> gens;
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> events;
+    // GT_CG_FUSED: the fused gridding of every problem, the CG vectors (r, p,
+    // q, mhd of every problem in one allocation), states and partial sums,
+    // and the per-problem table the batched CG kernels read
+    FusedGridding* fused = nullptr;
+    DevBuf fvec, fstate, fpart, fprobs;
+    int fused_item_len = 0;  // NC2C entries per work item the autotuner chose
     ~gt_cg_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto e : events) (void)hipEventDestroy(e);
         for (auto s : streams) (void)hipStreamDestroy(s);
+        if (fused) fused_destroy(fused);
     }
 };
 
@@ -897,6 +1006,86 @@ void tune_graph(gt_cg_batch* b, hipStream_t s) {
             (void)hipGraphDestroy(g);
         }
     }
+}
+
+// GT_CG_FUSED: the whole batch's solve, one launch per step for every problem
+// (per size group for the FFT passes); capturable (no allocation, no host read).
+void fused_solve(gt_cg_batch* b, hipStream_t s) {
+    const unsigned np = (unsigned)b->ops.size();
+    const CgProb* probs = b->fprobs.as<CgProb>();
+    const dim3 grid(kFusedRedBlocks, np);
+    // rhs = w_E E^H d, x = 0, r = p = rhs
+    fused_adjoint(b->fused, true, s);
+    hipLaunchKernelGGL(cg_coil_sum_batch_kernel, grid, dim3(kRedThreads), 0, s, probs);
+    hipLaunchKernelGGL(cg_init_batch_kernel, grid, dim3(kRedThreads), 0, s, probs);
+    hipLaunchKernelGGL(cg_init_finish_batch_kernel, dim3(1, np), dim3(kRedThreads), 0, s, probs);
+    GT_HIP(hipGetLastError());
+    for (int it = 0; it < b->desc.iterations; it++) {
+        fused_forward(b->fused, s);         // E p, every problem
+        fused_adjoint(b->fused, false, s);  // E^H E p before the coil sums
+        hipLaunchKernelGGL(cg_q_csm_batch_kernel, grid, dim3(kRedThreads), 0, s, probs);
+        hipLaunchKernelGGL(cg_xr_step_batch_kernel, grid, dim3(kRedThreads), 0, s, probs, it);
+        hipLaunchKernelGGL(cg_p_step_batch_kernel, grid, dim3(kRedThreads), 0, s, probs, it);
+        GT_HIP(hipGetLastError());
+    }
+}
+
+// Builds the fused gridding and the CG tables of a GT_CG_FUSED batch, then
+// records its solve into b->graph / b->exec.
+void fused_setup(gt_cg_batch* b, hipStream_t s) {
+    const int n = int(b->ops.size());
+    std::vector<size_t> off(size_t(n) + 1, 0);
+    for (int i = 0; i < n; i++) off[size_t(i) + 1] = off[size_t(i)] + b->ops[size_t(i)]->image_elems();
+    const size_t total = off[size_t(n)];
+    b->fvec.reserve(std::max<size_t>(4 * total, 1) * sizeof(float2));
+    b->fstate.reserve(size_t(n) * size_t(b->desc.iterations + 1) * sizeof(CgState));
+    b->fpart.reserve(size_t(n) * 2 * kFusedRedBlocks * sizeof(double2));
+    float2* v = b->fvec.as<float2>();
+    auto vec = [&](int i, int k) { return v + size_t(k) * total + off[size_t(i)]; };  // k: r, p, q, mhd
+    std::vector<FusedProblem> fp;
+    for (int i = 0; i < n; i++) {
+        gt_sense_op* op = b->ops[size_t(i)];
+        fp.push_back(FusedProblem{op->plan, op->C, op->dcw, op->dcw_len, op->csm, vec(i, 1), b->data[size_t(i)]});
+    }
+    if (b->fused) fused_destroy(b->fused);
+    b->fused = nullptr;
+    b->fused = fused_create(fp, s);
+    std::vector<CgProb> probs(static_cast<size_t>(n));
+    for (int i = 0; i < n; i++) {
+        gt_sense_op* op = b->ops[size_t(i)];
+        CgProb& d = probs[size_t(i)];
+        d.x = static_cast<float2*>(b->x[size_t(i)]);
+        d.r = vec(i, 0);
+        d.p = vec(i, 1);
+        d.q = vec(i, 2);
+        d.mhd = vec(i, 3);
+        d.coil = static_cast<const float2*>(fused_coil_images(b->fused, i));
+        d.csm = op->csm;
+        d.nimg = (long long)op->nimg;
+        d.F = int(op->F);
+        d.C = int(op->C);
+        d.P = cg_params(op, &b->desc);
+        d.st = b->fstate.as<CgState>() + size_t(i) * size_t(b->desc.iterations + 1);
+        d.part = b->fpart.as<double2>() + size_t(i) * 2 * kFusedRedBlocks;
+    }
+    b->fprobs.reserve(size_t(n) * sizeof(CgProb));
+    GT_HIP(hipMemcpyAsync(b->fprobs.ptr, probs.data(), size_t(n) * sizeof(CgProb), hipMemcpyHostToDevice, s));
+    GT_HIP(hipStreamSynchronize(s));
+    // record the solve on a stream of its own (the caller's may be the null stream)
+    if (b->streams.empty()) {
+        hipStream_t st;
+        GT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->streams.push_back(st);
+    }
+    hipStream_t cs = b->streams[0];
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    GT_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    fused_solve(b, cs);
+    GT_HIP(hipStreamEndCapture(cs, &b->graph));
+    GT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
 }
 
 }  // namespace
@@ -1005,7 +1194,11 @@ gt_status gt_cgsense_batch_create(gt_cg_batch** out, gt_sense_op* const* ops, co
         GT_REQUIRE(out && ops && data && x && nprob > 0, GT_ERR_INVALID_ARGUMENT, "null or empty problem set");
         *out = nullptr;
         validate_desc(desc);
-        GT_REQUIRE(mode == GT_CG_PLAIN || mode == GT_CG_BATCHED, GT_ERR_INVALID_ARGUMENT, "unknown batch mode");
+        GT_REQUIRE(mode == GT_CG_PLAIN || mode == GT_CG_BATCHED || mode == GT_CG_FUSED, GT_ERR_INVALID_ARGUMENT,
+                   "unknown batch mode");
+        GT_REQUIRE(mode != GT_CG_FUSED || (!desc->precond && desc->n_regs == 0), GT_ERR_UNSUPPORTED,
+                   "the fused batch takes an identity regulariser (kappa) only: no preconditioner or image "
+                   "regularisers");
         // one descriptor serves problems of different sizes: no initial guess
         // or regulariser images (a preconditioner must divide every image)
         GT_REQUIRE(!desc->x0, GT_ERR_UNSUPPORTED, "a batch takes no initial guess");
@@ -1031,10 +1224,16 @@ gt_status gt_cgsense_batch_create(gt_cg_batch** out, gt_sense_op* const* ops, co
             b->data.push_back(data[i]);
             b->x.push_back(x[i]);
         }
-        if (mode == GT_CG_BATCHED) {
-            tune_graph(b.get(), s);
-            for (auto* o : b->ops) b->gens.emplace_back(plan_generation(o->plan), o->generation);
+        if (mode == GT_CG_BATCHED) tune_graph(b.get(), s);
+        if (mode == GT_CG_FUSED) {
+            fused_setup(b.get(), s);
+            // two replays: the first pays any first-launch cost
+            GT_HIP(hipGraphLaunch(b->exec, s));
+            GT_HIP(hipGraphLaunch(b->exec, s));
+            GT_HIP(hipStreamSynchronize(s));
         }
+        if (mode != GT_CG_PLAIN)
+            for (auto* o : b->ops) b->gens.emplace_back(plan_generation(o->plan), o->generation);
         *out = b.release();
     });
 }
@@ -1043,7 +1242,7 @@ gt_status gt_cgsense_batch_run(gt_cg_batch* b, gt_stream_t stream) {
     return guarded([&] {
         GT_REQUIRE(b, GT_ERR_INVALID_ARGUMENT, "null batch");
         hipStream_t s = S(stream);
-        if (b->mode == GT_CG_BATCHED) {
+        if (b->mode == GT_CG_BATCHED || b->mode == GT_CG_FUSED) {
             for (size_t i = 0; i < b->ops.size(); i++)
                 GT_REQUIRE(b->gens[i].first == plan_generation(b->ops[i]->plan) &&
                                b->gens[i].second == b->ops[i]->generation,

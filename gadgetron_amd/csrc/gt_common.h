@@ -105,4 +105,34 @@ bool plan_is_double(const gt_nufft_plan* p);
 // workspace reallocation moves or rewrites state that launched kernels read.
 uint64_t plan_generation(const gt_nufft_plan* p);
 
+// ---- fused gridding of many 2D problems (the fused batched CG-SENSE) --------
+// Problems are grouped by (N, N_os).  Each group keeps one oversampled grid
+// [y][its batches][x] and one coil-image buffer [batch][N1][N0]; a stage of
+// every problem is one launch (gridding, split-tile reduction) or one launch
+// per group (the FFT passes), from per-problem argument tables built here.
+struct FusedProblem {
+    gt_nufft_plan* plan;   // preprocessed float 2D plan (not owned)
+    size_t C;              // coils; batches F C
+    const float* dcw;      // NULL or per-sample weights
+    size_t dcw_len;
+    const void* csm;       // [N..., C] complex64
+    const void* image;     // image the forward transform reads, [N..., F]
+    const void* data;      // samples the first adjoint reads, [M, F, C]
+};
+struct FusedGridding;
+// Why a problem cannot take part, or an empty string.
+std::string fused_unsupported(const FusedProblem& p);
+FusedGridding* fused_create(const std::vector<FusedProblem>& probs, hipStream_t s);
+void fused_destroy(FusedGridding* f);
+// E^H before the coil sum for every problem: samples (the callers' data, or
+// the internal samples of the last fused_forward) -> coil images.
+void fused_adjoint(FusedGridding* f, bool from_data, hipStream_t s);
+// E for every problem: image (coil maps applied in the first FFT pass) ->
+// internal samples.
+void fused_forward(FusedGridding* f, hipStream_t s);
+// Coil images of problem i after fused_adjoint, [N..., F, C] complex64.
+void* fused_coil_images(const FusedGridding* f, int i);
+// Generations of the bound plans, in problem order.
+std::vector<uint64_t> fused_generations(const FusedGridding* f);
+
 }  // namespace gt_host

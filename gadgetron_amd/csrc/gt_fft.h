@@ -122,16 +122,25 @@ __device__ __forceinline__ void dft<8>(float2 (&x)[8], float s) {
     }
 }
 
-// LDS slot of element i of a sequence: contiguous sequences (row transforms)
-// get one pad slot per 8 elements, so the stride-R writes of a pass spread
-// over the banks; interleaved sequences (columns) use a plain stride.
-template <bool PADR>
+// LDS slot of element i of a sequence.  Contiguous sequences (row
+// transforms, ROW) keep each aligned block of 16 elements (one pass of the 32
+// four-byte banks for 8-byte accesses) and permute it: the low four bits are
+// XORed with g(i / 16) = (i / 16) mod 8 + 8 ((i / 16) & 4 != 0).  Contiguous
+// reads stay one block per 16 lanes; the stride-8 writes of a first radix-8
+// pass (16 lanes over 8 blocks) and the split writes of a second (two runs of
+// 8, 4 blocks apart) reach 16 distinct banks pairs.  Interleaved sequences
+// (columns) use a plain stride.
+template <bool ROW>
 __device__ __forceinline__ int fft_slot(int i, int estride) {
-    return PADR ? i + (i >> 3) : i * estride;
+    if (ROW) {
+        const int r = i >> 4;
+        return i ^ ((r & 7) | ((r & 4) << 1));
+    }
+    return i * estride;
 }
-template <int L, bool PADR>
+template <int L, bool ROW>
 constexpr int fft_slots() {
-    return PADR ? L + L / 8 : L;
+    return ROW ? (L + 15) / 16 * 16 : L;
 }
 
 // One Stockham pass of radix R on sequences of length L held in LDS (element
@@ -201,6 +210,10 @@ struct FftGeom {
     // then reads image f times map c (the SENSE coil multiply)
     const float2* csm;
     int F;
+    // forwards, many problems: the image and coil map of batch b are
+    // img_b[b] and csm_b[b] (each [Ny][Nx]); null: img / csm above
+    const float2* const* img_b;
+    const float2* const* csm_b;
 };
 
 constexpr int kFftRowThreads = 64;  // threads per row transform
@@ -243,10 +256,18 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_pad_kernel(
     const int b = live ? r / fg.Ny : 0, iy = live ? r % fg.Ny : 0;
     const int y = iy + fg.offy;
     const float sy = fg.scale * fg.fy[y] * ((y & 1) ? -1.0f : 1.0f);
-    const bool coils = fg.csm != nullptr;
-    const int fb = coils ? b % fg.F : b;  // image batch read
-    const float2* in = img + ((long long)fb * fg.Ny + iy) * fg.Nx;
-    const float2* cm = coils ? fg.csm + ((long long)(b / fg.F) * fg.Ny + iy) * fg.Nx : nullptr;
+    const bool tables = fg.img_b != nullptr;  // block-uniform
+    const bool coils = tables ? fg.csm_b != nullptr : fg.csm != nullptr;
+    const float2* in;
+    const float2* cm = nullptr;
+    if (tables) {
+        in = fg.img_b[b] + (long long)iy * fg.Nx;
+        if (coils) cm = fg.csm_b[b] + (long long)iy * fg.Nx;
+    } else {
+        const int fb = coils ? b % fg.F : b;  // image batch read
+        in = img + ((long long)fb * fg.Ny + iy) * fg.Nx;
+        if (coils) cm = fg.csm + ((long long)(b / fg.F) * fg.Ny + iy) * fg.Nx;
+    }
     float2* sb = &sbuf[rl][0];
     // every load of the row before the first LDS store (one memory latency)
     constexpr int kPer = (OX + kFftRowThreads - 1) / kFftRowThreads;

@@ -27,6 +27,32 @@ namespace gt {
 constexpr int kRT = kTabTile;  // register tile edge
 constexpr int kWaves = 4;      // wavefronts per C2NC workgroup
 
+// Arguments of one problem's matrix-core 2D gridding (either direction): the
+// kernels below take them as launch arguments, and the batched launches of
+// many problems from a per-problem table.
+struct MfGridArgs {
+    const float* in;      // NC2C: samples [M, F, C]; C2NC: the grid
+    float* out;           // NC2C: the grid; C2NC: samples
+    float* partials;      // NC2C: partial tiles of split tiles
+    const float* wt;      // NC2C: table weights; C2NC: per-sample weight rows
+    const uint32_t* hd;   // NC2C: table heads; C2NC: the tile-sorted sample list
+    const WorkItem* items;
+    int M, F, C;
+    GridGeom g;
+    Layout L;
+    int sign;
+    const float* dcw;
+    long long dcw_len;
+    int accumulate;
+    float scale;          // NC2C: the weight normaliser; C2NC: the output scale
+};
+
+// One unit of a batched gridding launch: problem, work item, coil chunk
+// (prob < 0: nothing to do).
+struct GridUnit {
+    int prob, item, chunk, pad_;
+};
+
 // Wave-uniform value the compiler cannot prove uniform by itself.
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -240,11 +266,9 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
 // Sum of the partial tiles of split tiles, in segment order (4 interleaved
 // accumulators combined in a fixed order, so the result is repeatable).  One
 // thread per (coil, cell, re/im) element of a tile; grid (split tiles, C*128/256).
-__global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restrict__ partials,
-                                                            float* __restrict__ grid,
-                                                            const RedItem* __restrict__ red, int F, int C,
-                                                            GridGeom g, Layout L, int sign, int accumulate) {
-    const RedItem it = red[blockIdx.x];
+__device__ __forceinline__ void nc2c_reduce2d_run(const float* __restrict__ partials, float* __restrict__ grid,
+                                                  const RedItem it, int F, int C, const GridGeom& g,
+                                                  const Layout& L, int sign, int accumulate) {
     const int i = blockIdx.y * blockDim.x + threadIdx.x;
     constexpr int kTileFloats = kRT * kRT * 2;
     if (i >= C * kTileFloats) return;
@@ -270,6 +294,25 @@ __global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restr
     const long long o = 2 * ((long long)gx + L.sy * gy + L.sb * ((long long)cc * F + f)) + comp;
     if (accumulate) v += grid[o];
     grid[o] = v;
+}
+
+__global__ __launch_bounds__(256) void nc2c_reduce2d_kernel(const float* __restrict__ partials,
+                                                            float* __restrict__ grid,
+                                                            const RedItem* __restrict__ red, int F, int C,
+                                                            GridGeom g, Layout L, int sign, int accumulate) {
+    nc2c_reduce2d_run(partials, grid, red[blockIdx.x], F, C, g, L, sign, accumulate);
+}
+
+// Many problems: block (u, y) sums split tile u of the list (unit.item indexes
+// the problem's RedItem list) for coil elements y * 256 ...
+__global__ __launch_bounds__(256) void nc2c_reduce2d_batch_kernel(const MfGridArgs* __restrict__ probs,
+                                                                  const RedItem* const* __restrict__ reds,
+                                                                  const GridUnit* __restrict__ units) {
+    const GridUnit u = units[blockIdx.x];
+    if (u.prob < 0) return;
+    const MfGridArgs& a = probs[u.prob];
+    if ((int)(blockIdx.y * blockDim.x) >= a.C * kRT * kRT * 2) return;
+    nc2c_reduce2d_run(a.partials, a.out, reds[u.prob][u.item], a.F, a.C, a.g, a.L, a.sign, a.accumulate);
 }
 
 // MODE 0: no dcw, no accumulation; 1: one dcw weight per sample, no
@@ -565,11 +608,20 @@ __device__ __forceinline__ float mf_absmax(const mf_float8& x) {
 }
 
 template <int DM, int CW>
-__global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
-    const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
-    const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
-    int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
-    int accumulate, float wscale, int n_items, int nchunks) {
+__device__ __forceinline__ void nc2c_mfma2d_run(const MfGridArgs& a, int item, int chunk) {
+    const float* __restrict__ samples = a.in;
+    float* __restrict__ grid = a.out;
+    float* __restrict__ partials = a.partials;
+    const float* __restrict__ wt = a.wt;
+    const uint32_t* __restrict__ hd = a.hd;
+    const WorkItem* __restrict__ items = a.items;
+    const int M = a.M, F = a.F, C = a.C;
+    const GridGeom& g = a.g;
+    const Layout L = a.L;
+    const int sign = a.sign, accumulate = a.accumulate;
+    const float* __restrict__ dcw = a.dcw;
+    const long long dcw_len = a.dcw_len;
+    const float wscale = a.scale;
     static_assert(CW == 32 || CW == 16, "coils per wave");
     constexpr int LC = CW / 4;       // coils per lane in a chunk's sample loads
     constexpr int MB = 2 * CW / 32;  // 32-row blocks of (coil, re / im)
@@ -581,11 +633,6 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
     uint32_t* sh = reinterpret_cast<uint32_t*>(wT + Lds::Tab);  // [2][2 kNK] heads
     float* xs = reinterpret_cast<float*>(sh + Lds::Heads);      // [re / im][coil][XS] per entry
     const int lane = threadIdx.x;
-    // block b = (item / 8) * 8 nchunks + chunk * 8 + item % 8: the coil chunks of an
-    // item run on its XCD (b mod 8 = item mod 8), 8 blocks apart, and share its L2
-    const int bx = (int)blockIdx.x, grp = 8 * nchunks;
-    const int item = (bx / grp) * 8 + (bx & 7), chunk = (bx / 8) % nchunks;
-    if (item >= n_items) return;
     const WorkItem it = items[item];
     if (it.partial == kNoopItem) return;
     const int c0 = chunk * CW;
@@ -762,6 +809,32 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
     }
 }
 
+// Block b = (item / 8) * 8 nchunks + chunk * 8 + item % 8: the coil chunks of an
+// item run on its XCD (b mod 8 = item mod 8), 8 blocks apart, and share its L2.
+template <int DM, int CW>
+__global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_kernel(
+    const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
+    const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
+    int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
+    int accumulate, float wscale, int n_items, int nchunks) {
+    const int bx = (int)blockIdx.x, grp = 8 * nchunks;
+    const int item = (bx / grp) * 8 + (bx & 7), chunk = (bx / 8) % nchunks;
+    if (item >= n_items) return;
+    const MfGridArgs a{samples, grid, partials, wt, hd, items, M, F, C, g, L, sign, dcw, dcw_len, accumulate, wscale};
+    nc2c_mfma2d_run<DM, CW>(a, item, chunk);
+}
+
+// Many problems in one launch: block b runs unit b of the list, with the
+// arguments of the unit's problem.
+template <int DM, int CW>
+__global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_batch_kernel(
+    const MfGridArgs* __restrict__ probs, const GridUnit* __restrict__ units) {
+    const GridUnit u = units[blockIdx.x];
+    if (u.prob < 0) return;
+    const MfGridArgs a = probs[u.prob];
+    nc2c_mfma2d_run<DM, CW>(a, u.item, u.chunk);
+}
+
 // ---------------------------------------------------------------------------
 // c2nc_mfma2d_kernel: the 2D C2NC of 16 or 32 coils per workgroup (CW) on
 // the matrix cores.
@@ -785,10 +858,19 @@ __host__ __device__ constexpr int c2nc_mfma_lds_bytes(int KW, int CW) {
 
 // CW: coils per workgroup (32 or 16: two or one 32-row blocks).
 template <int KW, int CW>
-__global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
-    const float* __restrict__ grid, float* __restrict__ out, const float* __restrict__ wtab,
-    const uint32_t* __restrict__ list, const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g,
-    Layout L, int sign, const float* __restrict__ dcw, long long dcw_len, int accumulate, float oscale, int n_items, int nchunks) {
+__device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, int chunk) {
+    const float* __restrict__ grid = a.in;
+    float* __restrict__ out = a.out;
+    const float* __restrict__ wtab = a.wt;
+    const uint32_t* __restrict__ list = a.hd;
+    const WorkItem* __restrict__ items = a.items;
+    const int M = a.M, F = a.F, C = a.C;
+    const GridGeom& g = a.g;
+    const Layout L = a.L;
+    const int sign = a.sign, accumulate = a.accumulate;
+    const float* __restrict__ dcw = a.dcw;
+    const long long dcw_len = a.dcw_len;
+    const float oscale = a.scale;
     constexpr int E = kRT + KW - 1;  // staged cells per dimension
     constexpr int AS = c2nc_mfma_row(KW);
     constexpr int MB = 2 * CW / 32;  // 32-row blocks
@@ -798,11 +880,6 @@ __global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
     _Float16* Alo = Ahi + 2 * CW * AS;
     float* red = reinterpret_cast<float*>(Alo + 2 * CW * AS);  // four wave maxima
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // block b = (item / 8) * 8 nchunks + chunk * 8 + item % 8: the coil chunks of an
-    // item run on its XCD (b mod 8 = item mod 8), 8 blocks apart, and share its L2
-    const int bx = (int)blockIdx.x, grp = 8 * nchunks;
-    const int item = (bx / grp) * 8 + (bx & 7), chunk = (bx / 8) % nchunks;
-    if (item >= n_items) return;
     const WorkItem it = items[item];
     if (it.partial == kNoopItem) return;
     const int c0 = chunk * CW;
@@ -932,6 +1009,29 @@ __global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
             }
             *o = val;
         }
+}
+
+// Block b = (item / 8) * 8 nchunks + chunk * 8 + item % 8, as the NC2C kernel.
+template <int KW, int CW>
+__global__ __launch_bounds__(256) void c2nc_mfma2d_kernel(
+    const float* __restrict__ grid, float* __restrict__ out, const float* __restrict__ wtab,
+    const uint32_t* __restrict__ list, const WorkItem* __restrict__ items, int M, int F, int C, GridGeom g,
+    Layout L, int sign, const float* __restrict__ dcw, long long dcw_len, int accumulate, float oscale, int n_items,
+    int nchunks) {
+    const int bx = (int)blockIdx.x, grp = 8 * nchunks;
+    const int item = (bx / grp) * 8 + (bx & 7), chunk = (bx / 8) % nchunks;
+    if (item >= n_items) return;
+    const MfGridArgs a{grid, out, nullptr, wtab, list, items, M, F, C, g, L, sign, dcw, dcw_len, accumulate, oscale};
+    c2nc_mfma2d_run<KW, CW>(a, item, chunk);
+}
+
+template <int KW, int CW>
+__global__ __launch_bounds__(256) void c2nc_mfma2d_batch_kernel(const MfGridArgs* __restrict__ probs,
+                                                                const GridUnit* __restrict__ units) {
+    const GridUnit u = units[blockIdx.x];
+    if (u.prob < 0) return;
+    const MfGridArgs a = probs[u.prob];
+    c2nc_mfma2d_run<KW, CW>(a, u.item, u.chunk);
 }
 
 }  // namespace gt

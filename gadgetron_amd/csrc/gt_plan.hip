@@ -21,6 +21,7 @@
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1669,6 +1670,259 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
 }
 
 }  // namespace
+
+// ---- fused gridding of many 2D problems ------------------------------------
+struct gt_host::FusedGridding {
+    struct Group {
+        gt_nufft_plan* rep = nullptr;  // one plan of the group: filters, twiddles, geometry
+        size_t nb = 0;                 // batches of the group
+        DevBuf grid;                   // [Oy][nb][Ox]
+        DevBuf coil;                   // [nb][N1][N0]
+        DevBuf img_b, csm_b;           // per-batch image and coil-map pointers
+    };
+    std::vector<FusedProblem> probs;
+    std::vector<int> group_of;   // problem -> group
+    std::vector<size_t> boff;    // problem -> its first batch in the group
+    std::vector<Group> groups;
+    DevBuf samples;              // internal samples of every problem
+    std::vector<size_t> soff;    // problem -> its offset into samples (complex elements)
+    DevBuf nc2c_data, nc2c_int, c2nc;  // gt::MfGridArgs per problem
+    DevBuf nc2c_units, c2nc_units, red_units, reds;
+    int n_nc2c = 0, n_c2nc = 0, n_red = 0, red_parts = 0;
+    int dm = 0;                  // dcw mode of the NC2C launches
+    int KW = 0;
+};
+
+namespace {
+
+constexpr int kFusedCW = 16;  // coils per wave / workgroup of the fused gridding
+
+template <int KW>
+struct LaunchFusedC2NC {
+    static void run(const gt::MfGridArgs* args, const gt::GridUnit* units, int n, hipStream_t s) {
+        hipLaunchKernelGGL((gt::c2nc_mfma2d_batch_kernel<KW, kFusedCW>), dim3((unsigned)n), dim3(256),
+                           size_t(gt::c2nc_mfma_lds_bytes(KW, kFusedCW)), s, args, units);
+    }
+};
+
+// Units of one problem's matrix-core gridding in the order of its own launch
+// (block b = (item / 8) 8 chunks + chunk 8 + item % 8, so b mod 8 = item mod
+// 8: the chunks of an item share its XCD); a problem's run is a multiple of 8
+// long, so the order survives concatenation.
+void append_units(std::vector<gt::GridUnit>& u, int prob, int n_items, int nchunks) {
+    const int nblk = (n_items + 7) / 8 * 8 * nchunks, grp = 8 * nchunks;
+    for (int b = 0; b < nblk; b++) {
+        const int item = (b / grp) * 8 + (b & 7), chunk = (b / 8) % nchunks;
+        u.push_back(item < n_items ? gt::GridUnit{prob, item, chunk, 0} : gt::GridUnit{-1, 0, 0, 0});
+    }
+}
+
+}  // namespace
+
+std::string gt_host::fused_unsupported(const FusedProblem& q) {
+    const gt_nufft_plan* p = q.plan;
+    if (!p) return "null plan";
+    if (p->dbl || p->D != 2) return "the fused batch takes float 2D plans";
+    if (!(p->fast2d && p->fft_fused && fft2d_strategy() == 2))
+        return "the fused batch takes plans whose oversampled sizes are fused FFT lengths (96 ... 384)";
+    if (p->conv_type != GT_CONV_STANDARD) return "the fused batch takes STANDARD convolution";
+    if (!gridding_on_matrix_cores()) return "the fused batch runs on the matrix cores (GT_NUFFT_MFMA=0 is set)";
+    if (!(p->prep_c2nc && p->prep_nc2c) || !p->bc.wtab.ptr) return "plan not preprocessed for both directions";
+    if (q.C < 1 || !q.csm || !q.image || !q.data) return "missing coil maps, image or data";
+    return "";
+}
+
+gt_host::FusedGridding* gt_host::fused_create(const std::vector<FusedProblem>& probs, hipStream_t s) {
+    GT_REQUIRE(!probs.empty(), GT_ERR_INVALID_ARGUMENT, "no problems");
+    const bool with_dcw = probs[0].dcw != nullptr;
+    for (const auto& q : probs) {
+        const std::string why = fused_unsupported(q);
+        GT_REQUIRE(why.empty(), GT_ERR_UNSUPPORTED, why);
+        GT_REQUIRE((q.dcw != nullptr) == with_dcw, GT_ERR_UNSUPPORTED,
+                   "the fused batch takes a dcw for every problem or for none");
+        GT_REQUIRE(q.plan->KW == probs[0].plan->KW && q.plan->device == probs[0].plan->device, GT_ERR_UNSUPPORTED,
+                   "the fused batch takes one kernel width and one device");
+    }
+    {
+        std::vector<const gt_nufft_plan*> seen;
+        for (const auto& q : probs) seen.push_back(q.plan);
+        std::sort(seen.begin(), seen.end());
+        GT_REQUIRE(std::adjacent_find(seen.begin(), seen.end()) == seen.end(), GT_ERR_INVALID_ARGUMENT,
+                   "every problem of a fused batch needs a plan of its own (the split tiles' partial sums are "
+                   "the plan's)");
+    }
+    auto f = std::make_unique<FusedGridding>();
+    f->probs = probs;
+    f->KW = probs[0].plan->KW;
+    set_device(probs[0].plan);
+    const int n = int(probs.size());
+    std::map<std::array<size_t, 4>, int> gid;
+    f->group_of.resize(size_t(n));
+    f->boff.resize(size_t(n));
+    for (int i = 0; i < n; i++) {
+        gt_nufft_plan* p = probs[size_t(i)].plan;
+        const std::array<size_t, 4> key{p->N[0], p->N[1], p->Nos[0], p->Nos[1]};
+        auto it = gid.find(key);
+        if (it == gid.end()) {
+            it = gid.emplace(key, int(f->groups.size())).first;
+            f->groups.emplace_back();
+            f->groups.back().rep = p;
+        }
+        auto& g = f->groups[size_t(it->second)];
+        f->group_of[size_t(i)] = it->second;
+        f->boff[size_t(i)] = g.nb;
+        g.nb += p->F * probs[size_t(i)].C;
+    }
+    // per-batch image / coil-map pointers of each group's forward row FFT
+    for (size_t gi = 0; gi < f->groups.size(); gi++) {
+        auto& g = f->groups[gi];
+        const gt_nufft_plan* p = g.rep;
+        GT_REQUIRE(g.nb * p->N[1] < (size_t(1) << 31) && g.nb * size_t((p->Nos[0] + 15) / 16) < (size_t(1) << 31),
+                   GT_ERR_UNSUPPORTED, "too many batches for one launch");
+        g.grid.reserve(g.nb * p->G() * sizeof(float2));
+        g.coil.reserve(g.nb * p->Nimg() * sizeof(float2));
+        std::vector<const float2*> ib(g.nb), cb(g.nb);
+        for (int i = 0; i < n; i++) {
+            if (f->group_of[size_t(i)] != int(gi)) continue;
+            const FusedProblem& q = probs[size_t(i)];
+            const size_t F = q.plan->F, nimg = q.plan->Nimg();
+            for (size_t c = 0; c < q.C; c++)
+                for (size_t fr = 0; fr < F; fr++) {
+                    const size_t b = f->boff[size_t(i)] + c * F + fr;
+                    ib[b] = static_cast<const float2*>(q.image) + fr * nimg;
+                    cb[b] = static_cast<const float2*>(q.csm) + c * nimg;
+                }
+        }
+        g.img_b.reserve(g.nb * sizeof(void*));
+        g.csm_b.reserve(g.nb * sizeof(void*));
+        GT_HIP(hipMemcpyAsync(g.img_b.ptr, ib.data(), g.nb * sizeof(void*), hipMemcpyHostToDevice, s));
+        GT_HIP(hipMemcpyAsync(g.csm_b.ptr, cb.data(), g.nb * sizeof(void*), hipMemcpyHostToDevice, s));
+        GT_HIP(hipStreamSynchronize(s));
+    }
+    // internal samples
+    size_t total = 0;
+    f->soff.resize(size_t(n));
+    for (int i = 0; i < n; i++) {
+        f->soff[size_t(i)] = total;
+        total += probs[size_t(i)].plan->M * probs[size_t(i)].plan->F * probs[size_t(i)].C;
+    }
+    f->samples.reserve(std::max<size_t>(total, 1) * sizeof(float2));
+    // per-problem arguments and the unit lists
+    std::vector<gt::MfGridArgs> nd(static_cast<size_t>(n)), ni(static_cast<size_t>(n)), cn(static_cast<size_t>(n));
+    std::vector<gt::GridUnit> un, uc, ur;
+    std::vector<const gt::RedItem*> reds(size_t(n), nullptr);
+    bool all_per_sample = true;
+    size_t cmax = 1;
+    for (int i = 0; i < n; i++) {
+        const FusedProblem& q = probs[size_t(i)];
+        gt_nufft_plan* p = q.plan;
+        const auto& g = f->groups[size_t(f->group_of[size_t(i)])];
+        const int C = int(q.C);
+        const gt::Layout L = internal_layout(p, g.nb);
+        float* gridp = g.grid.as<float>() + 2 * f->boff[size_t(i)] * p->Nos[0];
+        float2* smp = f->samples.as<float2>() + f->soff[size_t(i)];
+        if (p->bn.n_partials > 0) p->partials.reserve(size_t(p->bn.n_partials) * q.C * p->bn.geom.tile_cells * sizeof(float2));
+        const long long dl = (long long)std::max<size_t>(q.dcw_len, 1);
+        if (q.dcw && q.dcw_len != p->M * p->F) all_per_sample = false;
+        nd[size_t(i)] = gt::MfGridArgs{static_cast<const float*>(q.data), gridp, p->partials.as<float>(),
+                                       p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(),
+                                       p->bn.items.as<gt::WorkItem>(), int(p->M), int(p->F), C, p->bn.geom, L, 1,
+                                       q.dcw, dl, 0, mfma_weight_scale(p)};
+        ni[size_t(i)] = nd[size_t(i)];
+        ni[size_t(i)].in = reinterpret_cast<const float*>(smp);
+        cn[size_t(i)] = gt::MfGridArgs{gridp, reinterpret_cast<float*>(smp), nullptr, p->bc.wtab.as<float>(),
+                                       p->bc.list.as<uint32_t>(), p->bc.items.as<gt::WorkItem>(), int(p->M),
+                                       int(p->F), C, p->bc.geom, L, 1, q.dcw, dl, 0,
+                                       1.0f / (kb_axis_scale(p, 0) * kb_axis_scale(p, 1))};
+        const int chunks = (C + kFusedCW - 1) / kFusedCW;
+        append_units(un, i, p->bn.n_items, chunks);
+        append_units(uc, i, p->bc.n_items, chunks);
+        reds[size_t(i)] = p->bn.red.as<gt::RedItem>();
+        for (int k = 0; k < p->bn.n_red; k++) ur.push_back(gt::GridUnit{i, k, 0, 0});
+        cmax = std::max(cmax, q.C);
+    }
+    f->dm = !with_dcw ? 0 : (all_per_sample ? 1 : 2);
+    f->n_nc2c = int(un.size());
+    f->n_c2nc = int(uc.size());
+    f->n_red = int(ur.size());
+    f->red_parts = int((cmax * gt::kRT * gt::kRT * 2 + 255) / 256);
+    auto upload = [&](DevBuf& d, const void* src, size_t bytes) {
+        d.reserve(std::max<size_t>(bytes, 1));
+        if (bytes) GT_HIP(hipMemcpyAsync(d.ptr, src, bytes, hipMemcpyHostToDevice, s));
+    };
+    upload(f->nc2c_data, nd.data(), nd.size() * sizeof(gt::MfGridArgs));
+    upload(f->nc2c_int, ni.data(), ni.size() * sizeof(gt::MfGridArgs));
+    upload(f->c2nc, cn.data(), cn.size() * sizeof(gt::MfGridArgs));
+    upload(f->nc2c_units, un.data(), un.size() * sizeof(gt::GridUnit));
+    upload(f->c2nc_units, uc.data(), uc.size() * sizeof(gt::GridUnit));
+    upload(f->red_units, ur.data(), ur.size() * sizeof(gt::GridUnit));
+    upload(f->reds, reds.data(), reds.size() * sizeof(void*));
+    GT_HIP(hipStreamSynchronize(s));
+    return f.release();
+}
+
+void gt_host::fused_destroy(FusedGridding* f) { delete f; }
+
+void gt_host::fused_forward(FusedGridding* f, hipStream_t s) {
+    for (auto& g : f->groups) {
+        gt_nufft_plan* p = g.rep;
+        const float scale = centring_global_sign(p) / std::sqrt(float(p->G()));
+        gt::FftGeom fg = fft_geom(p, g.nb, true, scale);
+        fg.F = 1;
+        fg.img_b = g.img_b.as<const float2*>();
+        fg.csm_b = g.csm_b.as<const float2*>();
+        float2* grid = g.grid.as<float2>();
+        dispatch_fft_len<LaunchRowsPad>(int(p->Nos[0]), (const float2*)nullptr, grid, fg, p->twiddle[0].as<float2>(),
+                                        -1.0f, s);
+        GT_HIP(hipGetLastError());
+        dispatch_fft_len<LaunchColsBandIn>(int(p->Nos[1]), grid, int(p->Nos[0]), fg, p->twiddle[1].as<float2>(), -1.0f,
+                                           s);
+        GT_HIP(hipGetLastError());
+    }
+    if (f->n_c2nc)
+        dispatch_fast<LaunchFusedC2NC>(f->KW, f->c2nc.as<gt::MfGridArgs>(), f->c2nc_units.as<gt::GridUnit>(),
+                                       f->n_c2nc, s);
+    GT_HIP(hipGetLastError());
+}
+
+void gt_host::fused_adjoint(FusedGridding* f, bool from_data, hipStream_t s) {
+    const gt::MfGridArgs* args = (from_data ? f->nc2c_data : f->nc2c_int).as<gt::MfGridArgs>();
+    if (f->n_nc2c) {
+        auto k = f->dm == 0 ? gt::nc2c_mfma2d_batch_kernel<0, kFusedCW>
+                            : (f->dm == 1 ? gt::nc2c_mfma2d_batch_kernel<1, kFusedCW>
+                                          : gt::nc2c_mfma2d_batch_kernel<2, kFusedCW>);
+        hipLaunchKernelGGL(k, dim3((unsigned)f->n_nc2c), dim3(64), 0, s, args, f->nc2c_units.as<gt::GridUnit>());
+        GT_HIP(hipGetLastError());
+    }
+    if (f->n_red) {
+        hipLaunchKernelGGL(gt::nc2c_reduce2d_batch_kernel, dim3((unsigned)f->n_red, (unsigned)f->red_parts), dim3(256),
+                           0, s, args, f->reds.as<const gt::RedItem*>(), f->red_units.as<gt::GridUnit>());
+        GT_HIP(hipGetLastError());
+    }
+    for (auto& g : f->groups) {
+        gt_nufft_plan* p = g.rep;
+        const float scale = centring_global_sign(p) / std::sqrt(float(p->G()));
+        const gt::FftGeom fg = fft_geom(p, g.nb, false, scale);
+        float2* grid = g.grid.as<float2>();
+        dispatch_fft_len<LaunchColsBandOut>(int(p->Nos[1]), grid, int(p->Nos[0]), fg, p->twiddle[1].as<float2>(),
+                                            1.0f, s);
+        GT_HIP(hipGetLastError());
+        dispatch_fft_len<LaunchRowsCrop>(int(p->Nos[0]), (const float2*)grid, g.coil.as<float2>(), fg,
+                                         p->twiddle[0].as<float2>(), 1.0f, s);
+        GT_HIP(hipGetLastError());
+    }
+}
+
+void* gt_host::fused_coil_images(const FusedGridding* f, int i) {
+    const auto& g = f->groups[size_t(f->group_of[size_t(i)])];
+    return g.coil.as<float2>() + f->boff[size_t(i)] * g.rep->Nimg();
+}
+
+std::vector<uint64_t> gt_host::fused_generations(const FusedGridding* f) {
+    std::vector<uint64_t> out;
+    for (const auto& q : f->probs) out.push_back(q.plan->generation);
+    return out;
+}
 
 // ---- fixed device scratch of gt_nufft_compute_noalloc -----------------------
 // The oversampled grid and the split tiles' partial sums of one call live here

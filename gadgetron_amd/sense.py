@@ -107,7 +107,10 @@ class CgBatch:
 
     mode "plain": one problem at a time with the dot products read back to the
     host each time (the reference's method); "batched": every solve captured
-    into one hipGraph with the problems on parallel branches."""
+    into one hipGraph with the problems on parallel branches; "fused": one
+    launch per step for every problem (gridding over all problems' work items,
+    FFT passes per size group, CG updates with a problem index), captured into
+    one hipGraph (2D plans, identity regulariser only)."""
 
     def __init__(self, ops, data, xs, iterations=10, tc_tolerance=1e-6, encoding_weight=1.0, kappa=0.0,
                  mode="batched", stream=None):
@@ -120,7 +123,7 @@ class CgBatch:
         hd = PA(*[int(a.data_ptr()) for a in data])
         hx = PA(*[int(a.data_ptr()) for a in xs])
         d = _desc(iterations, tc_tolerance, encoding_weight, kappa, None)
-        m = {"plain": 0, "batched": 1}[mode]
+        m = {"plain": 0, "batched": 1, "fused": 2}[mode]
         h = ctypes.c_void_p()
         check(lib().gt_cgsense_batch_create(ctypes.byref(h), hops, hd, hx, n, ctypes.byref(d), m, _stream(stream)))
         self._h = h.value

@@ -101,9 +101,20 @@ gt_status gt_cgsense_solve_image(gt_sense_op* op, const void* rhs, void* x, cons
  *   GT_CG_BATCHED: every problem's whole solve (all iterations) captured into
  *                  one hipGraph with the problems spread over parallel
  *                  branches (longest first), launched as one graph with no
- *                  host round trips.
- * Both modes compute the same arithmetic (results agree bit for bit). */
-enum { GT_CG_PLAIN = 0, GT_CG_BATCHED = 1 };
+ *                  host round trips;
+ *   GT_CG_FUSED:   every step of every problem in one launch, from
+ *                  per-problem descriptor tables: one gridding launch per
+ *                  direction over the work items of all problems, the FFT
+ *                  passes once per group of problems with equal matrix sizes,
+ *                  the CG updates with a problem index; all iterations
+ *                  captured into one hipGraph.  2D float plans with fused FFT
+ *                  lengths and the matrix-core gridding, an identity
+ *                  regulariser (kappa) only.  It holds one oversampled grid
+ *                  per size group for all its problems' coils.
+ * PLAIN and BATCHED compute the same arithmetic (results agree bit for bit);
+ * FUSED runs every problem's gridding on the matrix cores (the others do so
+ * from 9 / 17 coils), so its results agree with theirs to rounding. */
+enum { GT_CG_PLAIN = 0, GT_CG_BATCHED = 1, GT_CG_FUSED = 2 };
 typedef struct gt_cg_batch gt_cg_batch;
 /* gt_cgsense_batch_create (GT_CG_BATCHED) times candidate graphs by running
  * them: it writes every problem's x (whole solves), reads the data and runs

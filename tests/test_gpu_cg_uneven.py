@@ -7,7 +7,9 @@ Bars: the batched solver equals the plain per-problem solver bit for bit on all
 64 problems; the three smallest problems and the smallest one with 17 coils or
 more at N >= 192 (its gridding runs on the matrix cores) match the oracle's
 CG-SENSE (oracle/cgsense.py over the hoNFFT restatement, the same data) at
-relative L2 <= 1e-4.
+relative L2 <= 1e-4.  The fused solver (one launch per step for all problems,
+every problem's gridding on the matrix cores) matches the plain solver at
+relative L2 <= 1e-4 on every problem and the oracle on the same picks.
 """
 import sys
 
@@ -75,6 +77,16 @@ def test_uneven_batched_equals_plain_and_oracle(uneven):
     assert big, "the workload has a problem with 17 coils or more at N >= 192"
     picks.append(big[0])
     po.set_num_threads(16)
+    say("fused CG (one launch per step for all problems), 64 problems x 10 iterations")
+    xf = [DeviceArray.zeros((1, p["N"], p["N"]), np.complex64) for p in probs]
+    fused = ga.CgBatch(ops, data, xf, iterations=10, tc_tolerance=0.0, mode="fused")
+    fused.run()
+    ops[0].plan.synchronize()
+    got_f = [a.numpy() for a in xf]
+    worst = max(rel_l2(f, a) for f, a in zip(got_f, got_p))
+    say(f"fused vs plain: worst rel L2 over 64 problems {worst:.3e}")
+    assert all(np.all(np.isfinite(f)) for f in got_f)
+    assert worst <= 1e-4
     for i in picks:
         pr = probs[i]
         say(f"oracle CG-SENSE problem {i}: N {pr['N']}, {pr['spokes']} spokes, {pr['C']} coils")
@@ -83,5 +95,30 @@ def test_uneven_batched_equals_plain_and_oracle(uneven):
         oop = cg.SenseOp(oplan, pr["csm"], pr["dcw"])
         xo, _ = cg.cg_solve(oop, pr["data"], 10, tc_tolerance=0.0)
         err = rel_l2(got_p[i], xo)
-        say(f"problem {i}: rel L2 vs oracle {err:.3e}")
-        assert err <= 1e-4, (i, err)
+        errf = rel_l2(got_f[i], xo)
+        say(f"problem {i}: rel L2 vs oracle {err:.3e} (plain), {errf:.3e} (fused)")
+        assert err <= 1e-4 and errf <= 1e-4, (i, err, errf)
+    # a second run of the same batch gives the same result (the graph replays)
+    fused.run()
+    ops[0].plan.synchronize()
+    assert all(np.array_equal(a.numpy(), f) for a, f in zip(xf, got_f))
+
+
+def test_fused_refusals(uneven):
+    """GT_CG_FUSED takes an identity regulariser only: a preconditioner is
+    refused on the host, before any device work."""
+    import ctypes
+
+    from gadgetron_amd._lib import lib
+    from gadgetron_amd.sense import _desc
+    probs, ops, data, _ = uneven
+    say("fused batch refusals")
+    xs = [DeviceArray.zeros((1, p["N"], p["N"]), np.complex64) for p in probs[:2]]
+    w = DeviceArray.zeros((1,), np.complex64)
+    d = _desc(2, 0.0, 1.0, 0.0, w)
+    PA = ctypes.c_void_p * 2
+    h = ctypes.c_void_p()
+    st = lib().gt_cgsense_batch_create(ctypes.byref(h), PA(*[o._h for o in ops[:2]]),
+                                       PA(*[a.data_ptr() for a in data[:2]]), PA(*[a.data_ptr() for a in xs]), 2,
+                                       ctypes.byref(d), 2, None)
+    assert st == 5 and not h.value
