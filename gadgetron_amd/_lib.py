@@ -54,6 +54,7 @@ EXPORTED_SYMBOLS = [
     "gt_cgsense_batch_run",
     "gt_cgsense_batch_destroy",
     "gt_cgsense_batch_branches",
+    "gt_cgsense_batch_item_length",
     "gt_image_operator_compute",
     "gt_cgsense_precond_weights",
     # include/gt_radial.h
@@ -197,6 +198,7 @@ def lib():
         "gt_cgsense_batch_run": [P, P],
         "gt_cgsense_batch_destroy": [P],
         "gt_cgsense_batch_branches": [P, ctypes.POINTER(I)],
+        "gt_cgsense_batch_item_length": [P, ctypes.POINTER(I)],
         "gt_image_operator_compute": [P, S, I, FP, P, P],
         "gt_cgsense_precond_weights": [P, S, S, P, D, P, P],
         "gt_radial_trajectory_golden_ratio_2d": [P, U, U, U, U, I, P],

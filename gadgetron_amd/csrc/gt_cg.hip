@@ -18,8 +18,13 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -1088,6 +1093,107 @@ void fused_setup(gt_cg_batch* b, hipStream_t s) {
     GT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
 }
 
+// Autotuner of the fused batch: NC2C table entries per work item, applied to
+// every plan of the batch at once.  The per-plan rule (0) sizes items so that
+// one problem alone gives two items per CU; in a fused launch all problems fill
+// the chip together, and longer items mean fewer split tiles to reduce.  Each
+// candidate's graph is replayed once to warm up and twice timed; the fastest
+// is kept.  The choice is cached per problem set (matrix and grid sizes,
+// samples, frames, coils, iterations) for the process and, where the
+// environment variable GT_NUFFT_TUNE_CACHE names a file, across processes
+// (one "signature length" line per set).
+constexpr int kTuneLens[] = {0, 256, 512, 1024, 2048};
+
+uint64_t fused_signature(const gt_cg_batch* b) {
+    uint64_t h = 1469598103934665603ull;  // FNV-1a
+    auto mix = [&](uint64_t v) {
+        for (int k = 0; k < 8; k++) {
+            h ^= (v >> (8 * k)) & 0xffu;
+            h *= 1099511628211ull;
+        }
+    };
+    mix(uint64_t(b->desc.iterations));
+    for (const gt_sense_op* op : b->ops) {
+        size_t Nos[4] = {0, 0, 0, 0};
+        op->check(gt_nufft_get_matrix_size_os(op->plan, Nos));
+        mix(op->N[0]);
+        mix(op->N[1]);
+        mix(Nos[0]);
+        mix(Nos[1]);
+        mix(op->M);
+        mix(op->F);
+        mix(op->C);
+    }
+    return h;
+}
+
+std::mutex tune_mu;
+std::map<uint64_t, int>& tune_cache() {
+    static std::map<uint64_t, int> cache;
+    static bool loaded = false;
+    if (!loaded) {
+        loaded = true;
+        if (const char* path = std::getenv("GT_NUFFT_TUNE_CACHE")) {
+            std::ifstream in(path);
+            std::string sig;
+            int len = 0;
+            while (in >> sig >> len) {
+                char* end = nullptr;
+                const uint64_t v = std::strtoull(sig.c_str(), &end, 16);
+                if (end && *end == '\0') cache[v] = len;
+            }
+        }
+    }
+    return cache;
+}
+
+double time_graph(gt_cg_batch* b, hipStream_t s) {
+    GT_HIP(hipGraphLaunch(b->exec, s));
+    GT_HIP(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    GT_HIP(hipGraphLaunch(b->exec, s));
+    GT_HIP(hipGraphLaunch(b->exec, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void fused_tune(gt_cg_batch* b, hipStream_t s) {
+    const uint64_t sig = fused_signature(b);
+    int chosen = -1;
+    {
+        std::lock_guard<std::mutex> lock(tune_mu);
+        auto& c = tune_cache();
+        auto it = c.find(sig);
+        if (it != c.end()) chosen = it->second;
+    }
+    auto apply = [&](int len) {
+        for (gt_sense_op* op : b->ops) plan_retile_nc2c(op->plan, len, s);
+        fused_setup(b, s);
+        b->fused_item_len = len;
+    };
+    if (chosen >= 0) {
+        apply(chosen);
+        return;
+    }
+    double best = 1e300;
+    int best_len = 0;
+    for (int len : kTuneLens) {
+        apply(len);
+        const double t = time_graph(b, s);
+        if (t < best) {
+            best = t;
+            best_len = len;
+        }
+    }
+    if (b->fused_item_len != best_len) apply(best_len);
+    std::lock_guard<std::mutex> lock(tune_mu);
+    tune_cache()[sig] = best_len;
+    if (const char* path = std::getenv("GT_NUFFT_TUNE_CACHE")) {
+        std::ofstream out(path, std::ios::app);
+        out << std::hex << sig << std::dec << " " << best_len << "\n";
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1225,13 +1331,7 @@ gt_status gt_cgsense_batch_create(gt_cg_batch** out, gt_sense_op* const* ops, co
             b->x.push_back(x[i]);
         }
         if (mode == GT_CG_BATCHED) tune_graph(b.get(), s);
-        if (mode == GT_CG_FUSED) {
-            fused_setup(b.get(), s);
-            // two replays: the first pays any first-launch cost
-            GT_HIP(hipGraphLaunch(b->exec, s));
-            GT_HIP(hipGraphLaunch(b->exec, s));
-            GT_HIP(hipStreamSynchronize(s));
-        }
+        if (mode == GT_CG_FUSED) fused_tune(b.get(), s);
         if (mode != GT_CG_PLAIN)
             for (auto* o : b->ops) b->gens.emplace_back(plan_generation(o->plan), o->generation);
         *out = b.release();
@@ -1326,6 +1426,13 @@ gt_status gt_cgsense_batch_branches(const gt_cg_batch* b, int* branches) {
     return guarded([&] {
         GT_REQUIRE(b && branches, GT_ERR_INVALID_ARGUMENT, "null argument");
         *branches = b->branches;
+    });
+}
+
+gt_status gt_cgsense_batch_item_length(const gt_cg_batch* b, int* entries) {
+    return guarded([&] {
+        GT_REQUIRE(b && entries, GT_ERR_INVALID_ARGUMENT, "null argument");
+        *entries = b->mode == GT_CG_FUSED ? b->fused_item_len : -1;
     });
 }
 

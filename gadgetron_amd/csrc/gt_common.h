@@ -10,6 +10,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/gt_nufft.h"
 
@@ -134,5 +135,9 @@ void fused_forward(FusedGridding* f, hipStream_t s);
 void* fused_coil_images(const FusedGridding* f, int i);
 // Generations of the bound plans, in problem order.
 std::vector<uint64_t> fused_generations(const FusedGridding* f);
+// Rebuilds a preprocessed 2D plan's NC2C work items with `len` table entries
+// per item (0: the plan's own rule); the gridding result changes only in the
+// summation order of split tiles.
+void plan_retile_nc2c(gt_nufft_plan* p, int len, hipStream_t s);
 
 }  // namespace gt_host

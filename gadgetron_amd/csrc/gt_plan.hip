@@ -151,6 +151,7 @@ struct gt_nufft_plan {
     int KW = 0;
     int tile_req[4] = {0, 0, 0, 0};
     int max_items_req = 0;
+    int nc2c_len_req = 0;  // NC2C entries per work item chosen by a fused batch's tuner (0: the rule below)
     bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
     bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
     bool fast3d = false;     // the 3D hot path (register-tile NC2C, staged C2NC)
@@ -1104,6 +1105,7 @@ size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& count
         const size_t cap = size_t(p->fast2d ? gt::kC2NCSeg : gt::k3Seg);
         return p->max_items_req > 0 ? std::min(cap, size_t(p->max_items_req)) : cap;
     }
+    if (p->nc2c_len_req > 0) return size_t(p->nc2c_len_req);
     if (p->max_items_req > 0) return size_t(p->max_items_req);
     if (p->fast2d) {
         // the matrix-core NC2C spends less per entry: items up to 512 entries
@@ -1862,6 +1864,15 @@ gt_host::FusedGridding* gt_host::fused_create(const std::vector<FusedProblem>& p
 }
 
 void gt_host::fused_destroy(FusedGridding* f) { delete f; }
+
+void gt_host::plan_retile_nc2c(gt_nufft_plan* p, int len, hipStream_t s) {
+    GT_REQUIRE(p && p->fast2d && p->prep_nc2c, GT_ERR_INVALID_ARGUMENT, "plan not preprocessed on the 2D hot path");
+    if (p->nc2c_len_req == len) return;
+    set_device(p);
+    p->nc2c_len_req = len;
+    p->generation++;
+    build_nc2c(p, s);
+}
 
 void gt_host::fused_forward(FusedGridding* f, hipStream_t s) {
     for (auto& g : f->groups) {

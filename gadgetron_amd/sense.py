@@ -139,6 +139,14 @@ class CgBatch:
         check(lib().gt_cgsense_batch_branches(self._h, ctypes.byref(n)))
         return n.value
 
+    @property
+    def item_length(self):
+        """NC2C entries per work item the fused mode's autotuner chose (0: each
+        plan's own rule; -1 for the other modes)."""
+        n = ctypes.c_int()
+        check(lib().gt_cgsense_batch_item_length(self._h, ctypes.byref(n)))
+        return n.value
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):

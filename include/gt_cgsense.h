@@ -131,6 +131,13 @@ gt_status gt_cgsense_batch_destroy(gt_cg_batch* batch);
 /* Parallel branches of a batched graph, picked at creation by timing one
  * replay per candidate (2, 4, ..., 64, and one per problem); 0 for GT_CG_PLAIN. */
 gt_status gt_cgsense_batch_branches(const gt_cg_batch* batch, int* branches);
+/* GT_CG_FUSED: NC2C table entries per work item its autotuner chose for every
+ * bound plan (0: each plan's own rule; -1 for the other modes).  Creation tries
+ * 0, 256, 512, 1024 and 2048 (re-tiling the bound plans' NC2C work items,
+ * which changes only the summation order of split tiles) and keeps the
+ * fastest; the choice is cached per problem set for the process and, where
+ * GT_NUFFT_TUNE_CACHE names a file, across processes. */
+gt_status gt_cgsense_batch_item_length(const gt_cg_batch* batch, int* entries);
 
 /* cuImageOperator<T>::compute (operators/imageOperator.h:28-47 with the
  * histogram offset of operators/gpu/cuImageOperator.h:30-63): the real
