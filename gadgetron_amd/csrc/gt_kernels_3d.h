@@ -252,6 +252,251 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg3d_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// nc2c_mfma3d_kernel: the 3D NC2C on the matrix cores.  One wave owns one
+// 8 x 8 x 4 tile for 8 coils, 16 rows q: the real parts of the 8 coils, then
+// their imaginary parts.  The tile is a product grid[q][cell] = sum over
+// entries k of s[q][k] w[k][cell], w[k][cell] = wx_k[x] wy_k[y] wz_k[z]: each
+// chunk of 32 table entries is one K = 32 step of v_mfma_f32_16x16x32_f16 on
+// 16 blocks of 16 cells (block = plane z and rows 2p, 2p + 1), with the
+// split-fp16 products hi*hi + hi*lo + lo*hi and the power-of-two scales of
+// nc2c_mfma2d_kernel (relative error about 2^-22 per product, FP32 sums).  Per
+// chunk the products wy_k[y] wz_k[z] of all 32 (y, z) pairs are formed once in
+// LDS, so a block's B operand is 8 products with the lane's wx_k[x].  The
+// table, its chunks and the items are those of nc2c_reg3d_kernel, staged in
+// LDS transposed (entries contiguous); entry and product order are fixed, so
+// the result is bitwise repeatable.
+// Operand lane layouts (16x16x32 f16): A holds row l % 16 at k = 8 (l / 16) +
+// t, B column l % 16 at the same k; accumulator register r of lane l is row
+// 4 (l / 16) + r, column l % 16.
+// ---------------------------------------------------------------------------
+constexpr int k3MK = 32;         // table entries per chunk (the MFMA's K)
+constexpr int k3MS = k3MK + 4;   // floats per LDS row of entry-contiguous data
+struct Mf3Lds {
+    static constexpr int Tab = 2 * k3NW * k3MS;     // two chunks' weights [chunk][component][entry]
+    static constexpr int Heads = 2 * 2 * k3MK;      // two chunks' (j, rows) heads
+    static constexpr int Xs = 16 * k3MS;            // samples [row q][entry]
+    static constexpr int Wyz = k3T1 * k3T2 * k3MS;  // wy wz products [y * 4 + z][entry]
+    static constexpr int Loop = Tab + Heads + Xs + Wyz;
+    static constexpr int Out = 16 * (k3T0 * k3T1 + 1);  // write-out of one plane [q][cell]
+    static constexpr int Floats = ((Out > Loop ? Out : Loop) + 3) & ~3;
+};
+static_assert(k3NW % 4 == 0 && k3MK * k3NW / 4 <= 3 * kLanes, "a chunk's weights load as three float4 per lane");
+
+template <int DM>
+__global__ __launch_bounds__(64, 3) void nc2c_mfma3d_kernel(
+    const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
+    const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
+    int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
+    int accumulate, float wscale) {
+    constexpr int kF4 = k3NW / 4;  // float4 per table entry
+    constexpr int kOut = k3T0 * k3T1 + 1;
+    __shared__ float4 smem[Mf3Lds::Floats / 4];
+    float* tT = reinterpret_cast<float*>(smem);                     // [2][k3NW][k3MS]
+    uint32_t* sh = reinterpret_cast<uint32_t*>(tT + Mf3Lds::Tab);   // [2][2 k3MK]
+    float* xs = reinterpret_cast<float*>(sh + Mf3Lds::Heads);       // [16][k3MS]
+    float* wyz = xs + Mf3Lds::Xs;                                   // [32][k3MS]
+    float* xp = reinterpret_cast<float*>(smem);                     // write-out [16][kOut]
+    const int lane = threadIdx.x;
+    const WorkItem it = items[blockIdx.x];
+    if (it.partial == kNoopItem) return;
+    const int c0 = blockIdx.y * k3Coils;
+    const long long FM = (long long)F * M;
+    const uint32_t nch = (it.end - it.begin + k3MK - 1) / k3MK;
+    const int row = lane & 15, kh = lane >> 4;            // MFMA: row / column l % 16, entries 8 kh ..
+    const int cx = lane & 7, cyo = (lane & 15) >> 3;      // the lane's B column: x, y offset in a block
+    const int se = lane & 31, ch = lane >> 5;             // sample loads: entry, coil half (4 coils)
+    const float2* s2 = reinterpret_cast<const float2*>(samples);
+
+    mf_float4 acc[16];  // block = z * 4 + p
+#pragma unroll
+    for (int b = 0; b < 16; b++) acc[b] = mf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+    float R = 0.0f;  // sample scale (0: not set yet)
+
+    constexpr int kDV = DM == 2 ? 4 : 1;
+    float4 wreg[3];
+    uint32_t hreg = 0u;
+    float2 sv[4];
+    float dv[kDV];
+#pragma unroll
+    for (int u = 0; u < 4; u++) sv[u] = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < kDV; u++) dv[u] = 1.0f;
+
+    // entries [cb, cb + 32) of [begin, end), end > begin; entries past end
+    // read as sample 0 with zero weights (clamped addresses)
+    auto load = [&](uint32_t cb) {
+        const float4* w4 = reinterpret_cast<const float4*>(wt);
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const int f = lane + kLanes * t;
+            const uint32_t e = cb + (uint32_t)(f / kF4);
+            const float4 v = w4[(size_t)min(e, it.end - 1) * kF4 + f % kF4];
+            wreg[t] = (f < k3MK * kF4 && e < it.end) ? v : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        const uint32_t eh = cb + (uint32_t)(lane >> 1);
+        const uint32_t hv = hd[(size_t)min(eh, it.end - 1) * 2 + (lane & 1)];
+        hreg = eh < it.end ? hv : 0u;
+    };
+    auto stage = [&](int b) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const int f = lane + kLanes * t;
+            if (f < k3MK * kF4) {
+                const int e = f / kF4, c = 4 * (f % kF4);
+                float* d = tT + (b * k3NW + c) * k3MS + e;
+                d[0] = wreg[t].x;
+                d[k3MS] = wreg[t].y;
+                d[2 * k3MS] = wreg[t].z;
+                d[3 * k3MS] = wreg[t].w;
+            }
+        }
+        sh[b * 2 * k3MK + lane] = hreg;
+    };
+    auto issue = [&](int b) {
+        const uint32_t j = sh[b * 2 * k3MK + 2 * se];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int c = c0 + 4 * ch + u;
+            c = c < C ? c : c0;  // coils past the last one repeat a valid coil, never stored
+            sv[u] = s2[(long long)c * FM + j];
+            if constexpr (DM == 2) dv[u] = dcw[dcw_index(c, j, FM, dcw_len)];
+        }
+        if constexpr (DM == 1) dv[0] = dcw[j];
+    };
+    auto transpose = [&]() {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            float2 v = sv[u];
+            if constexpr (DM != 0) {
+                const float d = dv[DM == 2 ? u : 0];
+                v.x *= d;
+                v.y *= d;
+            }
+            const int cl = 4 * ch + u;
+            xs[cl * k3MS + se] = v.x;
+            xs[(k3Coils + cl) * k3MS + se] = v.y;
+        }
+    };
+    auto grid_chunk = [&](int b) {
+        const float* tb = tT + b * k3NW * k3MS;
+        {
+            // wy_k[y] wz_k[z] for pair (y, z) = lane / 2, entries 16 (lane & 1) ..
+            const int pr = lane >> 1, y = pr >> 2, z = pr & 3, k0 = 16 * (lane & 1);
+            const float4* py = reinterpret_cast<const float4*>(tb + (k3T0 + y) * k3MS + k0);
+            const float4* pz = reinterpret_cast<const float4*>(tb + (k3T0 + k3T1 + z) * k3MS + k0);
+            float4* d = reinterpret_cast<float4*>(wyz + pr * k3MS + k0);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float4 a = py[q], c = pz[q];
+                d[q] = make_float4(a.x * c.x, a.y * c.y, a.z * c.z, a.w * c.w);
+            }
+        }
+        __syncthreads();
+        // A: row `row`, entries 8 kh .. 8 kh + 7
+        const float Rt = R == 0.0f ? 1.0f : R;
+        const float4* xa = reinterpret_cast<const float4*>(xs + row * k3MS + 8 * kh);
+        const float4 a0 = xa[0], a1 = xa[1];
+        mf_float8 x = mf_float8{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w} * Rt;
+        const float m = mf_absmax(x);
+        if (__builtin_amdgcn_ballot_w64(m > 32768.0f || (R == 0.0f && m > 0.0f))) {
+            const float mx = mf_wave_max(m);  // of x = samples * Rt
+            int e;
+            frexpf(mx, &e);                   // mx < 2^e
+            e = min(max(e, -100), 140);       // keep R and 1 / R finite
+            const float fs = ldexpf(1.0f, 12 - e);
+            if (R != 0.0f) {
+#pragma unroll
+                for (int bl = 0; bl < 16; bl++) acc[bl] *= fs;  // powers of two: exact
+            }
+            x *= fs;
+            R = Rt * fs;
+        }
+        mf_half8 xh, xl;
+        mf_split(x, xh, xl);
+        // B: the lane's x weights for its 8 entries (scaled), times wy wz of each block
+        const float4* px = reinterpret_cast<const float4*>(tb + cx * k3MS + 8 * kh);
+        const float4 x0 = px[0], x1 = px[1];
+        const mf_float8 wx = mf_float8{x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w} * wscale;
+#pragma unroll
+        for (int z = 0; z < k3T2; z++) {
+#pragma unroll
+            for (int pp = 0; pp < 4; pp += 2) {
+                // two blocks at a time: the accumulators alternate between their products
+                mf_half8 bh[2], bl[2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int y = 2 * (pp + h) + cyo;
+                    const float4* pw = reinterpret_cast<const float4*>(wyz + (y * k3T2 + z) * k3MS + 8 * kh);
+                    const float4 w0 = pw[0], w1 = pw[1];
+                    mf_split(wx * mf_float8{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w}, bh[h], bl[h]);
+                }
+                const int b0 = z * 4 + pp, b1 = b0 + 1;
+                acc[b0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh[0], acc[b0], 0, 0, 0);
+                acc[b1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh[1], acc[b1], 0, 0, 0);
+                acc[b0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl[0], acc[b0], 0, 0, 0);
+                acc[b1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl[1], acc[b1], 0, 0, 0);
+                acc[b0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh[0], acc[b0], 0, 0, 0);
+                acc[b1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh[1], acc[b1], 0, 0, 0);
+            }
+        }
+    };
+
+    if (nch > 0) {
+        load(it.begin);
+        stage(0);
+        load(it.begin + k3MK);
+        __syncthreads();
+        issue(0);
+        for (uint32_t c = 0; c < nch; c++) {
+            const int b = (int)(c & 1u);
+            transpose();
+            stage(b ^ 1);
+            load(it.begin + (c + 2) * k3MK);
+            __syncthreads();
+            issue(b ^ 1);
+            grid_chunk(b);
+            __syncthreads();
+        }
+    }
+
+    // write-out through LDS one plane at a time: xp[q][y * 8 + x], then lanes
+    // run along (re/im, x) for coalesced stores of (coil, y) rows
+    const float outs = R > 0.0f ? 1.0f / (R * wscale) : 0.0f;  // powers of two: exact
+    const int f = (int)(it.key / (uint32_t)g.ntiles_total);
+    int org[3];
+    tile_origin3((int)(it.key % (uint32_t)g.ntiles_total), g, org);
+#pragma unroll
+    for (int z = 0; z < k3T2; z++) {
+        __syncthreads();
+#pragma unroll
+        for (int pp = 0; pp < 4; pp++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                xp[(4 * kh + r) * kOut + (2 * pp + cyo) * k3T0 + cx] = acc[z * 4 + pp][r] * outs;
+        __syncthreads();
+        const int gz = org[2] + z;
+#pragma unroll 4
+        for (int t = 0; t < 16; t++) {
+            const int idx = lane + kLanes * t;  // 8 coils x 8 rows x 8 columns x (re, im)
+            const int comp = idx & 1, lx = (idx >> 1) & 7, ly = (idx >> 4) & 7, cl = idx >> 7;
+            const int cc = c0 + cl, gx = org[0] + lx, gy = org[1] + ly;
+            if (cc >= C || gx >= g.nos[0] || gy >= g.nos[1] || gz >= g.nos[2]) continue;
+            float v = xp[(comp * k3Coils + cl) * kOut + ly * k3T0 + lx];
+            if (it.partial >= 0) {
+                const int cell = (z * k3T1 + ly) * k3T0 + lx;
+                partials[(((long long)it.partial * C + cc) * (k3T0 * k3T1 * k3T2) + cell) * 2 + comp] = v;
+                continue;
+            }
+            if (sign && ((gx + gy + gz) & 1)) v = -v;
+            const long long o =
+                2 * ((long long)gx + L.sy * gy + L.sz * gz + L.sb * ((long long)cc * F + f)) + comp;
+            if (accumulate) v += grid[o];
+            grid[o] = v;
+        }
+    }
+}
+
 // Sum of the partial tiles of split tiles in segment order; one thread per
 // (coil, cell, re/im) element of a tile.
 __global__ __launch_bounds__(256) void nc2c_reduce3d_kernel(const float* __restrict__ partials,

@@ -575,6 +575,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
 typedef _Float16 mf_half8 __attribute__((ext_vector_type(8)));
 typedef float mf_float8 __attribute__((ext_vector_type(8)));
 typedef float mf_float16 __attribute__((ext_vector_type(16)));
+typedef float mf_float4 __attribute__((ext_vector_type(4)));
 
 // CW: coils per wave (32: two 32-row blocks, real and imaginary parts; 16: one
 // block, rows = (re / im, coil)).

@@ -776,11 +776,30 @@ struct LaunchTablesC2NC3 {
     }
 };
 
+// Power of two that brings the largest product of the three 1-D kernel
+// weights (at u = 0) to at most 1: the weight normaliser of nc2c_mfma3d_kernel.
+float mfma_weight_scale3(const gt_nufft_plan* p) {
+    const gt::GridGeom& g = p->geom;
+    double peak = 1.0;
+    for (int d = 0; d < 3; d++) peak *= double(gt::kb1(0.0f, g.nos_f[d], g.one_over_W, g.beta[d]));
+    return std::ldexp(1.0f, -int(std::ceil(std::log2(peak))));
+}
+
 void launch_fast3_nc2c(const float* samples, float* grid, float* partials, const gt_nufft_plan* p, int C,
                        const gt::Layout& L, int sign, const float* dcw, long long dcw_len, int accumulate,
                        hipStream_t s) {
     const unsigned chunks = (unsigned)((C + gt::k3Coils - 1) / gt::k3Coils);
     const long long FM = (long long)(p->M * p->F);
+    if (gridding_on_matrix_cores()) {
+        // the tile products on the matrix cores (split fp16, FP32 sums)
+        auto km = !dcw ? gt::nc2c_mfma3d_kernel<0>
+                       : (dcw_len == FM ? gt::nc2c_mfma3d_kernel<1> : gt::nc2c_mfma3d_kernel<2>);
+        hipLaunchKernelGGL(km, dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s, samples, grid, partials,
+                           p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(), p->bn.items.as<gt::WorkItem>(),
+                           int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw, dcw_len, accumulate,
+                           mfma_weight_scale3(p));
+        return;
+    }
     auto k = !dcw ? gt::nc2c_reg3d_kernel<0> : (dcw_len == FM ? gt::nc2c_reg3d_kernel<1> : gt::nc2c_reg3d_kernel<2>);
     hipLaunchKernelGGL(k, dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s, samples, grid, partials,
                        p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(), p->bn.items.as<gt::WorkItem>(),
