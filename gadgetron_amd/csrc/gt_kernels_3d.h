@@ -17,7 +17,11 @@
 //   while the current one is gathered.  Each wavefront gathers four samples at
 //   a time: lane (sample slot, x half, coil) reads whole complex cells and
 //   sums its half of every x row of the window; the halves are added across
-//   lanes.
+//   lanes.  With the matrix cores off (GT_NUFFT_MFMA=0) or windows of 7 cells.
+//
+// nc2c_mfma3d_kernel, c2nc_mfma3d_kernel: the same two directions on the
+// matrix cores (split-fp16 products, FP32 sums), described where they are
+// defined below.
 #pragma once
 
 #include "gt_kernels_reg.h"
@@ -577,16 +581,18 @@ __global__ void c2nc_table3d_kernel(const float* __restrict__ p, const uint32_t*
 }
 
 // LDS of the C2NC kernel: one plane per coil holding the staged cells
-// E0 x E1 x E2 (tile + KW - 1 per dimension) as (re, im) pairs in the grid's
+// E0 x E1 x E2 (c2nc_extent per dimension) as (re, im) pairs in the grid's
 // own order (x fastest), then zeroed pad cells -- a window reads up to one cell,
 // row and plane past the staged tile (and, with the x halves, 2 XH - KW more
 // cells), always with weight 0.  The plane stride is 4 (mod 32) floats: the 8
 // coil lanes of one x half, reading (re, im), then fall on banks {4c, 4c + 1},
 // and those of the other half, 2 XH floats on (XH = 3 for W 5.5), on the
 // remaining 16 banks.
-__host__ __device__ constexpr int c2nc3_cells(int KW) { return (k3S0 + KW - 1) * (k3S1 + KW - 1) * (k3S2 + KW - 1); }
+__host__ __device__ constexpr int c2nc3_cells(int KW) {
+    return c2nc_extent(k3S0, KW) * c2nc_extent(k3S1, KW) * c2nc_extent(k3S2, KW);
+}
 __host__ __device__ constexpr int c2nc3_pad_cells(int KW) {
-    return (k3S0 + KW - 1) * (k3S1 + KW - 1) + (k3S0 + KW - 1) + 8;
+    return c2nc_extent(k3S0, KW) * c2nc_extent(k3S1, KW) + c2nc_extent(k3S0, KW) + 8;
 }
 __host__ __device__ constexpr int c2nc3_plane_floats(int KW) {
     return ((2 * (c2nc3_cells(KW) + c2nc3_pad_cells(KW)) + 31) / 32) * 32 + 4;
@@ -602,7 +608,7 @@ __host__ __device__ constexpr int c2nc3_lds_bytes(int KW) { return (c2nc3_lds_fl
 // row and plane.
 template <int KW>
 struct C2NC3Stage {
-    static constexpr int E0 = k3S0 + KW - 1, E1 = k3S1 + KW - 1, E2 = k3S2 + KW - 1;
+    static constexpr int E0 = c2nc_extent(k3S0, KW), E1 = c2nc_extent(k3S1, KW), E2 = c2nc_extent(k3S2, KW);
     static constexpr int NT = 64 * k3Waves;
     static constexpr int RF = 2 * E0;               // floats per staged row
     static constexpr int R = kLanes / RF;           // rows per load instruction
@@ -837,6 +843,395 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
                 issue(ch + 2);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// c2nc_mfma3d_kernel: the 3D C2NC of 8 coils per workgroup on the matrix cores.
+//
+// Samples are binned by home tile of T x T x T cells, T = c2nc_mfma3_tile(KW)
+// (3 for windows of 5 or 6 cells), so that a tile and its kernel halo span 8
+// staged cells per dimension (from tile origin - floor(r)).  For each staged plane z the samples of a tile are a
+// product,
+//   P_z[q][n] = sum over staged (x, y) of grid[q][x, y, z] * wx_n[x] * wy_n[y],
+// q = (coil, re/im), n = sample, and sample n of row q is sum_z wz_n[z] P_z[q][n].
+// A work item is a run of tiles along z in one column (tx, ty): the workgroup
+// keeps the 8 staged planes of the current tile in a ring of 8 LDS plane
+// slots, so the next tile loads only its T new planes.  A plane is staged by
+// one wave, scaled by its own power of two R_z (largest |value| below 2^12)
+// and split into fp16 hi / lo rows; each wave then takes 16 samples at a time
+// and forms P_z as two K = 32 steps of v_mfma_f32_16x16x32_f16 (lane group
+// l / 16 takes staged row y = 4 h + l / 16 of step h, 8 x values per lane)
+// with the three products hi*hi + hi*lo + lo*hi of nc2c_mfma2d_kernel.  The B
+// operand wx wy is split once per 16 samples and serves all 8 planes; the z
+// weights and 1 / R_z are applied in FP32.  Out: s * oscale (oscale undoes the
+// table's power-of-two x and y scales), then dcw.  Product order is fixed:
+// the result is bitwise repeatable.
+// ---------------------------------------------------------------------------
+constexpr int k3ME = 8;                             // staged cells per dimension
+constexpr int k3MAS = k3ME * k3ME * k3ME + 8;       // halves per LDS row (1040 bytes: 4 dwords past 32 banks)
+constexpr int k3MW = 24;                            // floats per weight-table entry
+constexpr int k3MWaves = 4;                         // wavefronts per workgroup
+constexpr int k3MSeg = 256;                         // entries per work item
+__host__ __device__ constexpr int c2nc_mfma3_tile(int KW) { return k3ME - c2nc_extent(0, KW); }
+
+// Tile keys of the matrix-core 3D C2NC: tz fastest along a column (tx, ty),
+// the columns in bricks of 4 x 4 (x fastest within a brick, bricks x fastest),
+// so that the columns staged at one time on an XCD are neighbours in x and y
+// and share their halo cells in its L2.  The key space is padded to whole
+// bricks (ntiles_total of this geometry counts the padding).
+__device__ __forceinline__ uint32_t column_tile3(int tx, int ty, int tz, const GridGeom& g) {
+    const int nbx = (g.ntiles[0] + 3) >> 2;
+    const int col = ((((ty >> 2) * nbx) + (tx >> 2)) << 4) | ((ty & 3) << 2) | (tx & 3);
+    return (uint32_t)col * (uint32_t)g.ntiles[2] + (uint32_t)tz;
+}
+
+__device__ __forceinline__ void column_tile_coords3(uint32_t t, const GridGeom& g, int& tx, int& ty, int& tz) {
+    const int nbx = (g.ntiles[0] + 3) >> 2;
+    tz = (int)(t % (uint32_t)g.ntiles[2]);
+    const int col = (int)(t / (uint32_t)g.ntiles[2]);
+    const int b = col >> 4;
+    tx = ((b % nbx) << 2) | (col & 3);
+    ty = ((b / nbx) << 2) | ((col >> 2) & 3);
+}
+
+// Home tile key of every sample in column order (the matrix-core 3D C2NC).
+__global__ void home_keys_column3_kernel(const float* __restrict__ p, long long nsamp, long long M, GridGeom g,
+                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nsamp) return;
+    float pw;
+    const int tx = home_tile_1d(p[j * 3 + 0], g, 0, pw);
+    const int ty = home_tile_1d(p[j * 3 + 1], g, 1, pw);
+    const int tz = home_tile_1d(p[j * 3 + 2], g, 2, pw);
+    keys[j] = (uint32_t)(j / M) * (uint32_t)g.ntiles_total + column_tile3(tx, ty, tz, g);
+    vals[j] = (uint32_t)j;
+}
+
+// Weight table of the matrix-core 3D C2NC, 24 floats per entry: the sample's x
+// weights at the 8 staged columns (scaled by sx), y weights at the 8 staged
+// rows (scaled by sy), z weights at the 8 staged planes; zero outside the
+// window.  sx and sy are powers of two that bring each axis's largest weight
+// to at most 1, so every x y product fits fp16.
+template <int KW>
+__global__ void c2nc_wtable3d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list, long long n,
+                                     GridGeom g, float sx, float sy, float* __restrict__ wtab) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    static_assert(KW >= 2, "a window of at least 2 cells");
+    const uint32_t j = list[e];
+    const int fr = (int)floorf(g.radius);
+    float r[3][k3ME];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        float pw;
+        const int td = home_tile_1d(p[(long long)j * 3 + d], g, d, pw);
+        float w[KW];
+        int lo;
+        window_raw<KW>(pw, g, d, w, lo);
+        // 0 <= b <= T; window slots past the 8 staged cells have weight 0
+        const int b = min(max(lo - (td * g.tile[d] - fr), 0), g.tile[d]);
+        const float sc = d == 0 ? sx : (d == 1 ? sy : 1.0f);
+#pragma unroll
+        for (int c = 0; c < k3ME; c++) r[d][c] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+#pragma unroll
+            for (int c = 0; c < k3ME; c++)
+                if (c == b + i) r[d][c] = w[i] * sc;
+    }
+    float4* t = reinterpret_cast<float4*>(wtab + e * k3MW);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        t[2 * d] = make_float4(r[d][0], r[d][1], r[d][2], r[d][3]);
+        t[2 * d + 1] = make_float4(r[d][4], r[d][5], r[d][6], r[d][7]);
+    }
+}
+
+// Largest of a non-negative value over the wavefront, in every lane: xor
+// swizzles within each 32-lane half, then the two halves through SGPRs (no
+// address registers, unlike a shuffle).
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (1 << 10) | 0x1f)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (2 << 10) | 0x1f)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (4 << 10) | 0x1f)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (8 << 10) | 0x1f)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (16 << 10) | 0x1f)));
+    return fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)));
+}
+
+// The column of a work item: frame, first tile, staging origin, and the
+// lane's staged column x = l % 8 of coil l / 8 (wrapped, with its parity).
+struct C2M3Column {
+    const float2* cb;  // the lane's coil and frame, staged column x, plane 0 row 0
+    int sy, sz0;       // staging origin in y and z (unwrapped) of the first tile
+    int px;            // parity of the lane's unwrapped x
+    bool ok;           // the lane's coil exists
+
+    __device__ __forceinline__ void setup(const float* __restrict__ grid, uint32_t key, int lane, int c0, int C,
+                                          int F, const GridGeom& g, const Layout& L) {
+        const int fr = (int)floorf(g.radius);
+        const int f = (int)(key / (uint32_t)g.ntiles_total);
+        int tx, ty, tz;
+        column_tile_coords3(key % (uint32_t)g.ntiles_total, g, tx, ty, tz);
+        const int x = lane & 7, cl = lane >> 3;
+        ok = c0 + cl < C;
+        // coordinates lie in [-floor(r), N_os + 8): one wrap each
+        int gx = tx * g.tile[0] - fr + x;
+        px = gx & 1;  // the parity of a wrapped coordinate equals that of the unwrapped one (even N_os)
+        gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
+        cb = reinterpret_cast<const float2*>(grid) + (L.sb * ((long long)(ok ? c0 + cl : c0) * F + f) + gx);
+        sy = ty * g.tile[1] - fr;
+        sz0 = tz * g.tile[2] - fr;
+    }
+};
+
+// One staged plane (unwrapped z coordinate zz) of a column in one wave: lane
+// = (x, coil) holds the plane's 8 rows y; a load instruction reads 64-byte
+// row segments.  The centring signs are kept as bits and applied when the
+// plane is split into LDS, so the loads stay in flight until then.
+struct C2M3Plane {
+    float2 v[k3ME];
+    uint32_t par;  // bit y: centring sign of row y
+
+    __device__ __forceinline__ void load(const C2M3Column& c, int zz, const GridGeom& g, const Layout& L, int sign) {
+        const int gz = zz < 0 ? zz + g.nos[2] : (zz >= g.nos[2] ? zz - g.nos[2] : zz);
+        const float2* pl = c.cb + L.sz * (long long)gz;
+        par = 0;
+#pragma unroll
+        for (int y = 0; y < k3ME; y++) {
+            int gy = c.sy + y;
+            const int p = (c.px + gy + zz) & 1;
+            gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
+            v[y] = c.ok ? pl[L.sy * (long long)gy] : make_float2(0.0f, 0.0f);
+            par |= (uint32_t)(sign && p) << y;
+        }
+    }
+
+    // scale by the plane's power of two R, centring signs, split into fp16 hi
+    // / lo rows 2 coil (re), 2 coil + 1 (im) of LDS plane slot `slot`; 1 / R
+    // goes to inv[slot]
+    __device__ __forceinline__ void store(_Float16* Ahi, _Float16* Alo, float* inv, int slot, int lane) const {
+        float m = 0.0f;
+#pragma unroll
+        for (int y = 0; y < k3ME; y++) m = fmaxf(m, fmaxf(fabsf(v[y].x), fabsf(v[y].y)));
+        const float mx = wave_max_nonneg(m);
+        int e2;
+        frexpf(mx, &e2);               // mx < 2^e2, e2 <= 128 for finite values
+        e2 = max(e2, -100);            // denormal planes: keep R and 1 / R normal
+        const float R = mx > 0.0f ? ldexpf(1.0f, 12 - e2) : 1.0f;
+        if (lane == 0) inv[slot] = 1.0f / R;  // a power of two: exact
+        const int x = lane & 7, cl = lane >> 3;
+        _Float16* hr = Ahi + (2 * cl) * k3MAS + 64 * slot + x;
+        _Float16* lr = Alo + (2 * cl) * k3MAS + 64 * slot + x;
+#pragma unroll
+        for (int y = 0; y < k3ME; y++) {
+            const float sc = ((par >> y) & 1) ? -R : R;
+            const float re = v[y].x * sc, im = v[y].y * sc;
+            const _Float16 rh = (_Float16)re, ih = (_Float16)im;
+            hr[8 * y] = rh;
+            lr[8 * y] = (_Float16)(re - (float)rh);
+            hr[k3MAS + 8 * y] = ih;
+            lr[k3MAS + 8 * y] = (_Float16)(im - (float)ih);
+        }
+    }
+};
+
+// One block of 16 samples' weights (lane = sample l % 16, row group l / 16),
+// with the dcw of the sample's entry (dcw_e: per entry, or null).
+struct C2M3Weights {
+    float4 x0, x1, z0, z1;
+    float wy0, wy1, d;
+    uint32_t j;
+    bool on;
+
+    __device__ __forceinline__ void load(const float* __restrict__ wtab, const uint32_t* __restrict__ list,
+                                         const float* __restrict__ dcw_e, uint32_t e0, uint32_t ne, uint32_t bk,
+                                         int n, int kg) {
+        const uint32_t e = bk * 16 + (uint32_t)n;
+        on = e < ne;
+        const uint32_t ee = e0 + (on ? e : 0u);  // ne > 0 here
+        const float* wr = wtab + (size_t)ee * k3MW;
+        const float4* w4 = reinterpret_cast<const float4*>(wr);
+        x0 = w4[0];
+        x1 = w4[1];
+        z0 = w4[4];
+        z1 = w4[5];
+        wy0 = wr[8 + kg];
+        wy1 = wr[12 + kg];
+        j = list[ee];
+        d = dcw_e ? dcw_e[ee] : 1.0f;
+    }
+};
+
+// Work-item record of the matrix-core 3D C2NC, one dword per lane (64):
+//   [0] first tile's key, [1] number of tiles nt (0: nothing to do),
+//   [2 + t] first entry of tile t (t <= nt; [2 + nt] = end), entries of the
+//   item's tiles being contiguous in entry order.
+constexpr int k3MRec = 64;
+constexpr int k3MMaxTiles = k3MRec - 3;
+
+// MODE 0: no dcw, no accumulation; 1: one dcw weight per sample (dcw_e: per
+// table entry), no accumulation; 2: anything else.
+// Persistent: workgroup b takes items b, b + gridDim.x, ... (gridDim.x a
+// multiple of 8 keeps item i on XCD i mod 8, as the items are ordered for).
+// While a tile is gathered, the next tile's new planes (one per wave) and the
+// record of the item after the current one are in flight; a tile's first
+// block of weights per wave is loaded before its planes are split into LDS,
+// and the gather itself loads nothing unless a tile has more blocks than
+// waves (or MODE 2).  Blocks go to the waves in reverse order, so that a tile
+// of fewer new planes than waves keeps every wave busy.  A column's first
+// tile has 8 new planes, of which the last four are loaded when the tile is
+// staged.
+// Three waves per SIMD: registers for the plane and weights in flight (LDS
+// would hold four workgroups per CU).
+template <int MODE>
+__global__ __launch_bounds__(64 * k3MWaves, 3) void c2nc_mfma3d_kernel(
+    const float* __restrict__ grid, float* __restrict__ out, const float* __restrict__ wtab,
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ recs, int n_items, int M, int F, int C,
+    GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
+    const float* __restrict__ dcw_e, int accumulate, float oscale) {
+    __shared__ float4 smem[(2 * 16 * k3MAS * 2 + k3ME * 4 + 15) / 16];
+    _Float16* Ahi = reinterpret_cast<_Float16*>(smem);
+    _Float16* Alo = Ahi + 16 * k3MAS;
+    float* inv = reinterpret_cast<float*>(Alo + 16 * k3MAS);  // 1 / R of each plane slot
+    const int lane = threadIdx.x & 63, wave = wave_uniform(threadIdx.x >> 6);
+    const int c0 = blockIdx.y * k3Coils;
+    const int n = lane & 15, kg = lane >> 4;
+    const int T = g.tile[2];
+    // blocks of 16 samples go to the waves in reverse order (wave 3 first):
+    // the new planes of a tile go to the first waves
+    const uint32_t wb = (uint32_t)(k3MWaves - 1 - wave);
+    // A: row q = l % 16, staged cells (x = 0..7, y = 4 h + kg) of slot s at 64 s + 32 h + 8 kg
+    const _Float16* ah = Ahi + n * k3MAS + 8 * kg;
+    const _Float16* al = Alo + n * k3MAS + 8 * kg;
+    const long long FM = (long long)F * M;
+    float2* o2 = reinterpret_cast<float2*>(out);
+    const float* de = MODE == 1 ? dcw_e : nullptr;
+    const int stride = (int)gridDim.x;
+    auto load_rec = [&](int i) -> uint32_t { return i < n_items ? recs[(size_t)i * k3MRec + lane] : 0u; };
+    auto rec_at = [](uint32_t r, int k) -> uint32_t { return (uint32_t)__builtin_amdgcn_readlane((int)r, k); };
+    // planes of tile t of a column: t = 0 all 8 from sz0, then T new ones from sz0 + 8 + T (t - 1)
+    auto first_plane = [&](const C2M3Column& c, int t) { return t == 0 ? c.sz0 : c.sz0 + k3ME + T * (t - 1); };
+
+    C2M3Plane pf;  // this wave's first new plane of the step ahead
+    // issues the loads of tile t of the column c
+    auto prefetch = [&](const C2M3Column& c, int t) {
+        const int np = t == 0 ? k3ME : T;
+        if (wave < np) pf.load(c, first_plane(c, t) + wave, g, L, sign);
+    };
+
+    // the current item (first with tiles), and the record of the next one
+    int item = (int)blockIdx.x;
+    uint32_t rc = load_rec(item);
+    while (item < n_items && rec_at(rc, 1) == 0u) {
+        item += stride;
+        rc = load_rec(item);
+    }
+    if (item >= n_items) return;  // uniform over the workgroup; no barrier was passed
+    uint32_t rn = load_rec(item + stride);
+    C2M3Column col;
+    col.setup(grid, rec_at(rc, 0), lane, c0, C, F, g, L);
+    int t = 0;
+    prefetch(col, 0);
+    while (true) {
+        const int np = t == 0 ? k3ME : T;
+        const int zb = first_plane(col, t);
+        const uint32_t e0 = rec_at(rc, 2 + t), ne = rec_at(rc, 3 + t) - e0;
+        C2M3Weights w0;
+        if (wb * 16 < ne) w0.load(wtab, list, de, e0, ne, wb, n, kg);
+        // split the new planes into their slots (every wave is past the previous gather)
+        if (wave < np) pf.store(Ahi, Alo, inv, (zb + wave) & 7, lane);
+        for (int j = wave + k3MWaves; j < np; j += k3MWaves) {
+            C2M3Plane q;
+            q.load(col, zb + j, g, L, sign);
+            q.store(Ahi, Alo, inv, (zb + j) & 7, lane);
+        }
+        __syncthreads();
+        // the step after this one: the next tile of this item, or the next item's first
+        int nitem = item, nt = t + 1;
+        uint32_t nrc = rc, nrn = rn;
+        C2M3Column ncol = col;
+        if (nt >= (int)rec_at(rc, 1)) {
+            nitem = item + stride;
+            nt = 0;
+            nrc = rn;
+            while (nitem < n_items && rec_at(nrc, 1) == 0u) {  // no-op items pad the ends of the XCD runs
+                nitem += stride;
+                nrc = load_rec(nitem);
+            }
+            if (nitem < n_items) {
+                nrn = load_rec(nitem + stride);
+                ncol.setup(grid, rec_at(nrc, 0), lane, c0, C, F, g, L);
+            }
+        }
+        const bool more = nitem < n_items;
+        if (more) prefetch(ncol, nt);
+
+        // gather: staged plane z of this tile is slot (tz0 + z) & 7
+        const int tz0 = col.sz0 + T * t;
+        const uint32_t nb = (ne + 15) / 16;  // blocks of 16 samples
+        for (uint32_t bk = wb; bk < nb; bk += k3MWaves) {
+            C2M3Weights w = w0;
+            if (bk != wb) w.load(wtab, list, de, e0, ne, bk, n, kg);  // the first was loaded before
+            const mf_float8 wx =
+                mf_float8{w.x0.x, w.x0.y, w.x0.z, w.x0.w, w.x1.x, w.x1.y, w.x1.z, w.x1.w} * (w.on ? 1.0f : 0.0f);
+            const float wz[k3ME] = {w.z0.x, w.z0.y, w.z0.z, w.z0.w, w.z1.x, w.z1.y, w.z1.z, w.z1.w};
+            mf_half8 bh0, bl0, bh1, bl1;
+            mf_split(wx * w.wy0, bh0, bl0);
+            mf_split(wx * w.wy1, bh1, bl1);
+            mf_float4 acc = mf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+            // an offset of 0 the compiler cannot see through, renewed per plane:
+            // the A rows are read from LDS per block and plane rather than
+            // held in 128 registers
+            int ao = 0;
+#pragma unroll
+            for (int z = 0; z < k3ME; z++) {
+                asm volatile("" : "+v"(ao));
+                const int sl = (tz0 + z) & 7;
+                const int o = ao + 64 * sl;
+                const mf_half8 a0h = *reinterpret_cast<const mf_half8*>(ah + o);
+                const mf_half8 a0l = *reinterpret_cast<const mf_half8*>(al + o);
+                const mf_half8 a1h = *reinterpret_cast<const mf_half8*>(ah + o + 32);
+                const mf_half8 a1l = *reinterpret_cast<const mf_half8*>(al + o + 32);
+                mf_float4 pz = mf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0l, bh0, pz, 0, 0, 0);
+                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, bl0, pz, 0, 0, 0);
+                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, bh0, pz, 0, 0, 0);
+                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, bh1, pz, 0, 0, 0);
+                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bl1, pz, 0, 0, 0);
+                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bh1, pz, 0, 0, 0);
+                acc += (wz[z] * inv[sl]) * pz;  // column l % 16 = this lane's sample
+            }
+            if (!w.on) continue;
+            // accumulator register r: row 4 kg + r = coil 2 kg + r / 2, re / im r % 2
+            float d = oscale * w.d;
+            const int dm = MODE == 2 ? dcw_mode(dcw, dcw_len, FM) : 0;
+            if (MODE == 2 && dm == 1) d *= dcw[w.j];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int cc = c0 + 2 * kg + h;
+                if (cc >= C) continue;
+                float dd = d;
+                if (MODE == 2 && dm == 2) dd *= dcw[dcw_index(cc, w.j, FM, dcw_len)];
+                float2 val = make_float2(acc[2 * h] * dd, acc[2 * h + 1] * dd);
+                float2* o = o2 + (long long)cc * FM + w.j;
+                if (MODE == 2 && accumulate) {
+                    const float2 old = *o;
+                    val.x += old.x;
+                    val.y += old.y;
+                }
+                *o = val;
+            }
+        }
+        if (!more) break;  // uniform; the LDS is not written again
+        __syncthreads();   // the gather is done with the planes the next tile overwrites
+        item = nitem;
+        t = nt;
+        rc = nrc;
+        rn = nrn;
+        col = ncol;
     }
 }
 

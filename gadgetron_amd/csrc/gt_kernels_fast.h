@@ -39,6 +39,15 @@ __device__ __forceinline__ uint32_t pack_entry(int bx, int by, uint32_t mx, uint
     return (uint32_t)(bx & 0xff) | ((uint32_t)(by & 0xff) << 8) | ((mx & 0xff) << 16) | ((my & 0xff) << 24);
 }
 
+// Staged cells per dimension of a C2NC tile of T cells plus the kernel halo.
+// Staged offsets start at tile origin - floor(r); a home sample's window
+// starts at offset b = 0..T, and its cells of nonzero weight end at offset
+// b + 2 floor(r) when r is not an integer (b - 1 + 2r when it is).  With
+// KW = floor(2r) + 1 slots, 2 floor(r) is KW - 1 for odd KW (r in [k, k + 1/2))
+// and KW - 2 for even KW, so the last cell of nonzero weight is at offset
+// T + KW - 1 (odd KW) or T + KW - 2 (even KW).
+__host__ __device__ constexpr int c2nc_extent(int T, int KW) { return T + KW - 1 + (KW & 1); }
+
 // Window of one sample along one dimension with unwrapped cell indices:
 // slots i = 0..KW-1 are cells lo + i, valid (weight != 0) up to floor(p + r).
 template <int KW>
@@ -128,7 +137,7 @@ __global__ void c2nc_wtable2d_kernel(const float* __restrict__ p, const uint32_t
                                      GridGeom g, float sx, float sy, float* __restrict__ wtab) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    static_assert(8 + KW - 1 <= 16, "a staged row holds 16 cells");
+    static_assert(c2nc_extent(8, KW) <= 16, "a staged row holds 16 cells");
     const uint32_t j = list[e];
     const int fr = (int)floorf(g.radius);
     float wx[KW], wy[KW];
@@ -244,11 +253,10 @@ __device__ __forceinline__ int dcw_mode(const float* dcw, long long dcw_len, lon
     return !dcw ? 0 : (dcw_len == FM ? 1 : 2);
 }
 
-// LDS floats of the C2NC staged tile: (T1 + KW - 1) rows of (T0 + KW - 1)
-// cells, then one zeroed row and one zeroed cell.  A home sample's window
-// starts at staged offset 0..T in each dimension (KW - 1 = floor(2r) >=
-// 2 floor(r)), so reading all KW x KW slots reaches at most one row and one
-// cell past the staged tile, always in slots of weight 0.
+// LDS floats of the C2NC staged tile: c2nc_extent(T1) rows of c2nc_extent(T0)
+// cells, then one zeroed row and one zeroed cell: reading all KW x KW slots
+// from offset b <= T reaches at most one row and one cell past the staged
+// tile (even KW), always in slots of weight 0.
 // Rounded up to whole float4s: the work item's table entries follow it.
 // A staged cell holds the 2 x 32 / G floats of a wave's coils plus one (an
 // odd stride keeps rows conflict-free).
@@ -260,7 +268,7 @@ __host__ __device__ constexpr int c2nc_cell_stride(int G) {
 }
 
 __host__ __device__ inline int c2nc_lds_floats(int T0, int T1, int KW, int G = 1) {
-    return (((T1 + KW) * (T0 + KW - 1) + 1) * c2nc_cell_stride(G) + 3) & ~3;
+    return (((c2nc_extent(T1, KW) + 1) * c2nc_extent(T0, KW) + 1) * c2nc_cell_stride(G) + 3) & ~3;
 }
 
 // C2NC work items hold at most this many entries (their table is staged in LDS).

@@ -334,7 +334,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(1, 
     static_assert(G == 1 || G == 2 || G == 4, "sample slots");
     constexpr int CW = kCoilsPerWave / G;  // coils per wave
     constexpr int SP = c2nc_cell_stride(G);  // LDS floats per staged cell
-    constexpr int E0 = kRT + KW - 1, E1 = kRT + KW - 1;
+    constexpr int E0 = c2nc_extent(kRT, KW), E1 = c2nc_extent(kRT, KW);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = wave_uniform(tid >> 6);
     const WorkItem it = items[blockIdx.x];
@@ -843,7 +843,7 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_batch_kernel
 // The samples of a home tile are a product: s[q][n] = sum over staged cells k
 // of grid[q][k] * w[k][n], q = (coil, re/im), n = sample, w[k][n] = wy_n[y] *
 // wx_n[x].  A workgroup of four waves stages the tile + halo (E x E cells, E =
-// 8 + KW - 1, rows padded to 16 cells) once, scaled by a power of two R and
+// c2nc_extent(8, KW), rows padded to 16 cells) once, scaled by a power of two R and
 // split into fp16 hi / lo planes in LDS; each wave then takes 32 samples and
 // walks the E staged rows as K = 16 steps of v_mfma_f32_32x32x16_f16 on
 // 2 CW / 32 blocks of 32 rows (16 coils each, rows 2 coil + re/im), with the three
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_batch_kernel
 // weights at staged coordinates (c2nc_wtable2d_kernel).  Out: s * oscale / R
 // (oscale undoes the table's power-of-two weight scales), then dcw.
 // ---------------------------------------------------------------------------
-__host__ __device__ constexpr int c2nc_mfma_row(int KW) { return (kRT + KW - 1) * 16 + 8; }  // halves per A row
+__host__ __device__ constexpr int c2nc_mfma_row(int KW) { return c2nc_extent(kRT, KW) * 16 + 8; }  // halves per A row
 __host__ __device__ constexpr int c2nc_mfma_lds_bytes(int KW, int CW) {
     return 2 * (2 * CW) * c2nc_mfma_row(KW) * 2 + 64;
 }
@@ -872,7 +872,7 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
     const float* __restrict__ dcw = a.dcw;
     const long long dcw_len = a.dcw_len;
     const float oscale = a.scale;
-    constexpr int E = kRT + KW - 1;  // staged cells per dimension
+    constexpr int E = c2nc_extent(kRT, KW);  // staged cells per dimension
     constexpr int AS = c2nc_mfma_row(KW);
     constexpr int MB = 2 * CW / 32;  // 32-row blocks
     static_assert(E <= 16 && (CW == 32 || CW == 16), "staged rows of 16 cells, 16 or 32 coils");

@@ -123,6 +123,7 @@ struct Binning {
     DevBuf wtab;      // per-entry weight rows of the matrix-core 2D C2NC
     DevBuf items;     // gt::WorkItem
     DevBuf red;       // gt::RedItem (split tiles)
+    DevBuf recs;      // work-item records of the matrix-core 3D C2NC
     size_t n = 0;
     int n_items = 0, n_red = 0, n_partials = 0;
     void release() {
@@ -133,6 +134,7 @@ struct Binning {
         heads.release();
         items.release();
         red.release();
+        recs.release();
         n = 0;
         n_items = n_red = n_partials = 0;
     }
@@ -155,6 +157,7 @@ struct gt_nufft_plan {
     bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
     bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
     bool fast3d = false;     // the 3D hot path (register-tile NC2C, staged C2NC)
+    bool c2nc3_mfma = false; // fast3d with the matrix-core C2NC (home tiles of (9 - KW)^3 cells)
     bool dbl = false;        // GT_DOUBLE: the generic path in double precision
     double radius_d = 0, one_over_W_d = 0, beta_d[4] = {0, 0, 0, 0}, nos_d[4] = {1, 1, 1, 1};
 
@@ -237,7 +240,8 @@ struct gt_nufft_plan {
         for (DevBuf* b : {&traj, &traj_d, &row_inv, &sparse_dcw, &flag, &grid, &tmp_samples, &partials})
             b->epoch = &generation;
         for (Binning* bb : {&bn, &bc})
-            for (DevBuf* b : {&bb->list, &bb->table, &bb->heads, &bb->dcw_e, &bb->wtab, &bb->items, &bb->red})
+            for (DevBuf* b : {&bb->list, &bb->table, &bb->heads, &bb->dcw_e, &bb->wtab, &bb->items, &bb->red,
+                              &bb->recs})
                 b->epoch = &generation;
     }
 
@@ -768,6 +772,15 @@ struct LaunchTablesNC2C3 {
 };
 
 template <int KW>
+struct LaunchWTableC2NC3 {
+    static void run(const float* pos, const uint32_t* list, long long n, const gt::GridGeom& g, float sx, float sy,
+                    float* wtab, hipStream_t s) {
+        hipLaunchKernelGGL((gt::c2nc_wtable3d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos,
+                           list, n, g, sx, sy, wtab);
+    }
+};
+
+template <int KW>
 struct LaunchTablesC2NC3 {
     static void run(const float* pos, const uint32_t* list, long long n, const gt::GridGeom& g, uint32_t* tab,
                     hipStream_t s) {
@@ -814,6 +827,27 @@ struct LaunchFast3C2NC {
         const size_t lds = size_t(gt::c2nc3_lds_bytes(KW));
         const int mode = accumulate ? 2 : (!dcw ? 0 : (dcw_len == (long long)(p->M * p->F) ? 1 : 2));
         Binning& b = p->bc;
+        if (p->c2nc3_mfma) {
+            // persistent: as many workgroups as are resident at once (LDS and
+            // registers), a multiple of the 8 XCDs so that every item stays on
+            // the XCD its position assigns
+            const float oscale = 1.0f / (kb_axis_scale(p, 0) * kb_axis_scale(p, 1));  // powers of two
+            auto km = mode == 0 ? gt::c2nc_mfma3d_kernel<0>
+                                : (mode == 1 ? gt::c2nc_mfma3d_kernel<1> : gt::c2nc_mfma3d_kernel<2>);
+            int per_cu = 0;
+            GT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, km, 64 * gt::k3MWaves, 0));
+            const int resident = std::max(8, (std::max(per_cu, 1) * p->cus) & ~7);
+            const unsigned ctas = (unsigned)std::min(((b.n_items + 7) / 8) * 8, resident);
+            if (mode == 1) {
+                b.dcw_e.reserve(std::max<size_t>(b.n, 1) * sizeof(float));
+                hipLaunchKernelGGL(gt::entry_dcw_kernel, dim3((unsigned)((b.n + 255) / 256)), dim3(256), 0, s,
+                                   b.list.as<uint32_t>(), (long long)b.n, dcw, b.dcw_e.as<float>());
+            }
+            hipLaunchKernelGGL(km, dim3(ctas, chunks), dim3(64 * gt::k3MWaves), 0, s, grid, out, b.wtab.as<float>(),
+                               b.list.as<uint32_t>(), b.recs.as<uint32_t>(), b.n_items, int(p->M), int(p->F), C,
+                               b.geom, L, sign, dcw, dcw_len, b.dcw_e.as<float>(), accumulate, oscale);
+            return;
+        }
         if (mode == 1) {
             b.dcw_e.reserve(std::max<size_t>(b.n, 1) * sizeof(float));
             hipLaunchKernelGGL(gt::entry_dcw_kernel, dim3((unsigned)((b.n + 255) / 256)), dim3(256), 0, s,
@@ -910,6 +944,7 @@ void setup_geometry(gt_nufft_plan* p) {
     // 3D hot path: STANDARD gridding, a window of 3..7 cells, no requested
     // tile, and a grid of at least tile + window cells per dimension.
     p->fast3d = false;
+    p->c2nc3_mfma = false;
     if (p->D == 3 && !p->dbl && p->conv_type != GT_CONV_ATOMIC && p->KW >= 3 && p->KW <= gt::kTabKW &&
         p->tile_req[0] <= 0 && p->tile_req[1] <= 0 && p->tile_req[2] <= 0) {
         const int tn[3] = {gt::k3T0, gt::k3T1, gt::k3T2}, tc[3] = {gt::k3S0, gt::k3S1, gt::k3S2};
@@ -922,7 +957,17 @@ void setup_geometry(gt_nufft_plan* p) {
         if (ok) {
             p->fast3d = true;
             p->bn.geom = make_geom(p, tn, false);
-            p->bc.geom = make_geom(p, tc, false);
+            // the matrix-core C2NC stages tile + halo as 8 cells per dimension:
+            // tiles of 3 cells for windows of 5 or 6 cells, 5 for 3 or 4
+            const int tm = gt::c2nc_mfma3_tile(p->KW);
+            const int tmc[3] = {tm, tm, tm};
+            p->c2nc3_mfma = gridding_on_matrix_cores() && tm >= 3;
+            p->bc.geom = make_geom(p, p->c2nc3_mfma ? tmc : tc, false);
+            if (p->c2nc3_mfma) {
+                // keys in column order (columns in bricks of 4 x 4): the key space is padded to whole bricks
+                gt::GridGeom& g = p->bc.geom;
+                g.ntiles_total = 16 * ((g.ntiles[0] + 3) / 4) * ((g.ntiles[1] + 3) / 4) * g.ntiles[2];
+            }
         }
     }
 }
@@ -1006,6 +1051,9 @@ std::vector<uint32_t> bin_home(gt_nufft_plan* p, const gt::GridGeom& g, Binning&
     vals.reserve(nsamp * 4);
     if (p->D == 2)
         hipLaunchKernelGGL(gt::home_keys_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
+                           p->traj.as<float>(), nsamp, (long long)p->M, g, keys.as<uint32_t>(), vals.as<uint32_t>());
+    else if (p->c2nc3_mfma)
+        hipLaunchKernelGGL(gt::home_keys_column3_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
                            p->traj.as<float>(), nsamp, (long long)p->M, g, keys.as<uint32_t>(), vals.as<uint32_t>());
     else
         hipLaunchKernelGGL(gt::home_keys_nd_kernel, dim3((unsigned)((nsamp + 255) / 256)), dim3(256), 0, s,
@@ -1114,6 +1162,69 @@ void make_items(gt_nufft_plan* p, Binning& b, const std::vector<uint32_t>& count
     GT_HIP(hipStreamSynchronize(s));
 }
 
+// Work items of the matrix-core 3D C2NC: runs of consecutive non-empty tiles
+// of one column segment -- the column's tiles in one z slab of kSlab tiles --
+// holding at most L entries and k3MMaxTiles tiles; a tile above L entries
+// becomes single-tile items of L entries.  Items run slab by slab, the
+// columns of a slab in key order (bricks of 4 x 4 columns), so the columns
+// staged at one time on an XCD are x and y neighbours at the same z and share
+// their halo cells in its L2.  Items are XCD-ordered in that order and stored
+// as records of k3MRec dwords (key, tiles, tile entry bounds).
+void make_column_items(Binning& b, const std::vector<uint32_t>& counts, size_t L, hipStream_t s) {
+    constexpr uint32_t kSlab = 8;
+    const uint32_t ntz = uint32_t(b.geom.ntiles[2]);
+    const uint32_t per_frame = uint32_t(b.geom.ntiles_total), ncol = per_frame / ntz;
+    const uint32_t nf = uint32_t(counts.size() / per_frame);
+    std::vector<uint32_t> tb(counts.size() + 1, 0);
+    for (size_t k = 0; k < counts.size(); k++) tb[k + 1] = tb[k] + counts[k];
+    std::vector<gt::WorkItem> items;  // key, begin, end, number of tiles
+    for (uint32_t f = 0; f < nf; f++)
+        for (uint32_t z0 = 0; z0 < ntz; z0 += kSlab)
+            for (uint32_t c = 0; c < ncol; c++) {
+                const uint32_t kb = f * per_frame + c * ntz;
+                uint32_t k = kb + z0;
+                const uint32_t ke = kb + std::min(ntz, z0 + kSlab);
+                while (k < ke) {
+                    const uint32_t n = counts[k];
+                    if (n == 0) {
+                        k++;
+                        continue;
+                    }
+                    if (n > L) {
+                        for (uint32_t b0 = tb[k]; b0 < tb[k + 1]; b0 += uint32_t(L))
+                            items.push_back({k, b0, std::min<uint32_t>(tb[k + 1], b0 + uint32_t(L)), 1});
+                        k++;
+                        continue;
+                    }
+                    // extend along the column while the tiles are non-empty and fit
+                    uint32_t e = k + 1, tot = n;
+                    while (e < ke && counts[e] > 0 && tot + counts[e] <= L && e - k < uint32_t(gt::k3MMaxTiles))
+                        tot += counts[e++];
+                    items.push_back({k, tb[k], tb[e], int32_t(e - k)});
+                    k = e;
+                }
+            }
+    items = xcd_order(std::move(items), false);
+    std::vector<uint32_t> rec(std::max<size_t>(items.size(), 1) * gt::k3MRec, 0u);
+    for (size_t i = 0; i < items.size(); i++) {
+        const gt::WorkItem& w = items[i];
+        if (w.partial == gt::kNoopItem) continue;  // 0 tiles
+        uint32_t* r = rec.data() + i * gt::k3MRec;
+        r[0] = w.key;
+        r[1] = uint32_t(w.partial);
+        r[2] = w.begin;
+        for (int t = 1; t < w.partial; t++) r[2 + t] = tb[w.key + uint32_t(t)];
+        r[2 + w.partial] = w.end;
+    }
+    b.n_items = int(std::max<size_t>(items.size(), 1));
+    b.n_red = 0;
+    b.n_partials = 0;
+    b.items.release();
+    b.recs.reserve(rec.size() * sizeof(uint32_t));
+    GT_HIP(hipMemcpyAsync(b.recs.ptr, rec.data(), rec.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    GT_HIP(hipStreamSynchronize(s));
+}
+
 // Entries per work item.  A wavefront walks its entries serially, so the
 // longest item bounds the kernel time; splitting an NC2C tile costs one
 // partial tile write and read, splitting a C2NC tile one more halo staging.
@@ -1121,7 +1232,7 @@ size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& count
                       bool nc2c) {
     if ((p->fast2d || p->fast3d) && !nc2c) {
         // the C2NC kernels stage a work item's whole table in LDS
-        const size_t cap = size_t(p->fast2d ? gt::kC2NCSeg : gt::k3Seg);
+        const size_t cap = size_t(p->fast2d ? gt::kC2NCSeg : (p->c2nc3_mfma ? gt::k3MSeg : gt::k3Seg));
         return p->max_items_req > 0 ? std::min(cap, size_t(p->max_items_req)) : cap;
     }
     if (p->nc2c_len_req > 0) return size_t(p->nc2c_len_req);
@@ -1195,13 +1306,24 @@ void build_c2nc(gt_nufft_plan* p, hipStream_t s) {
         } else {
             b.wtab.release();
         }
+    } else if (p->c2nc3_mfma) {
+        b.table.release();
+        b.wtab.reserve(std::max<size_t>(b.n, 1) * gt::k3MW * sizeof(float));
+        const float sx = kb_axis_scale(p, 0), sy = kb_axis_scale(p, 1);
+        if (b.n)
+            dispatch_fast<LaunchWTableC2NC3>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
+                                             b.geom, sx, sy, b.wtab.as<float>(), s);
     } else {
+        b.wtab.release();
         b.table.reserve(std::max<size_t>(b.n, 1) * gt::k3CDw * 4);
         dispatch_fast<LaunchTablesC2NC3>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
                                          b.geom, b.table.as<uint32_t>(), s);
     }
     GT_HIP(hipGetLastError());
-    make_items(p, b, counts, segment_length(p, counts, b.n, false), false, s);
+    if (p->c2nc3_mfma)
+        make_column_items(b, counts, segment_length(p, counts, b.n, false), s);
+    else
+        make_items(p, b, counts, segment_length(p, counts, b.n, false), false, s);
 }
 
 // ---- compute building blocks ----------------------------------------------

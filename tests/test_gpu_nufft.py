@@ -155,6 +155,30 @@ def test_coil_groups_2d(gpu, C, dcw_kind):
     assert rel_l2(d_g2.numpy().reshape(-1), ref) <= TOL
 
 
+@pytest.mark.parametrize("W,C", [(4.5, 8), (4.5, 32), (6.5, 32), (4.0, 32), (3.5, 8)])
+def test_window_widths_2d(gpu, W, C):
+    """2D gridding at other kernel widths, both directions: a window of KW =
+    floor(W) + 1 slots whose radius W / 2 has a fraction below 1/2 (4.5, 6.5)
+    reaches one staged cell further past the home tile than one whose
+    fraction is 1/2 or more (3.5, 5.5) or 0 (4.0); 8 coils take the FP32
+    C2NC, 32 the matrix cores."""
+    say(f"2D 64^2 W={W}, C={C}, per-sample dcw")
+    rng = np.random.default_rng(300 + C + int(10 * W))
+    N = [64, 64]
+    traj = tj.golden_radial_2d(128, 90)
+    F, M = traj.shape[0], traj.shape[1]
+    plan = gpu_plan(N, W=W)
+    plan.preprocess(dev(traj))
+    op = oracle_plan(N, traj, W=W)
+    smp = tj.complex_normal(rng, (C, F, M))
+    img = tj.complex_normal(rng, (C, F, 64, 64))
+    dcw = tj.ramp_dcf(traj)
+    ea = rel_l2(run(plan, smp, img.shape, CM.BACKWARDS_NC2C, dcw), op.compute(smp, img.shape, po.BACKWARDS_NC2C, dcw=dcw))
+    ef = rel_l2(run(plan, img, smp.shape, CM.FORWARDS_C2NC, dcw), op.compute(img, smp.shape, po.FORWARDS_C2NC, dcw=dcw))
+    say(f"  rel err adjoint {ea:.2e} forward {ef:.2e}")
+    assert ea <= TOL and ef <= TOL
+
+
 def test_standalone_convolve_fft_deapodize(gpu):
     say("standalone convolve / fft / deapodize")
     rng = np.random.default_rng(21)

@@ -53,14 +53,14 @@ def conv(plan, x, out_shape, mode, acc=None):
     return dout.numpy()
 
 
-def check_both_ways(N, traj, C, dcw, seed, max_items=0):
+def check_both_ways(N, traj, C, dcw, seed, max_items=0, W=5.5):
     F, M = traj.shape[0], traj.shape[1]
     rng = np.random.default_rng(seed)
-    plan = NFFTPlan(N, None, W=5.5, os_factor=1.5)
+    plan = NFFTPlan(N, None, W=W, os_factor=1.5)
     if max_items:
         plan.set_tiling(None, max_items)
     plan.preprocess(dev(traj))
-    op = po.Plan(N, 5.5, alpha=1.5, precision="float")
+    op = po.Plan(N, W, alpha=1.5, precision="float")
     op.preprocess(traj)
     smp = tj.complex_normal(rng, (C, F, M))
     img = tj.complex_normal(rng, (C, F) + tuple(N[::-1]))
@@ -97,6 +97,15 @@ def test_3d_split_tiles(gpu):
     say("3D 32^3, 8 coils: tiles split into 40-entry items (partials + reduce)")
     traj = tj.kooshball_3d(64, 400, seed=14)
     check_both_ways([32, 32, 32], traj, 8, tj.ramp_dcf(traj, power=2), seed=75, max_items=40)
+
+
+@pytest.mark.parametrize("W", [3.5, 4.5, 6.5])
+def test_3d_window_widths(gpu, W):
+    # windows of 4 and 5 cells: matrix-core C2NC tiles of 5^3 and 4^3 cells
+    # (more new planes per tile than waves); 7 cells: the staged FP32 C2NC
+    say(f"3D 32 x 28 x 36, 8 coils, W {W}, per-sample dcw")
+    traj = tj.kooshball_3d(64, 260, seed=16)
+    check_both_ways([32, 28, 36], traj, 8, tj.ramp_dcf(traj, power=2), seed=77, W=W)
 
 
 def test_3d_nc2c_repeatable_and_matches_generic(gpu):
