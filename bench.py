@@ -123,11 +123,13 @@ def cpu_baseline(wl, samples, budget_s):
     }
 
 
-# 17 coils or more: the 2D gridding runs on the matrix cores unless GT_NUFFT_MFMA=0
-_FP32_2D = os.environ.get("GT_NUFFT_MFMA", "1").startswith("0")
-KERNELS_2D = {"nc2c": ("nc2c_reg2d_kernel" if _FP32_2D else "nc2c_mfma2d_kernel") + " + nc2c_reduce2d_kernel",
-              "c2nc": "c2nc_multi2d_kernel" if _FP32_2D else "c2nc_mfma2d_kernel"}
-KERNELS_3D = {"nc2c": "nc2c_reg3d_kernel + nc2c_reduce3d_kernel", "c2nc": "c2nc_multi3d_kernel"}
+# 17 coils or more: the 2D gridding runs on the matrix cores unless GT_NUFFT_MFMA=0 (in 3D at
+# every coil count, for windows of up to 6 cells)
+_FP32_GRIDDING = os.environ.get("GT_NUFFT_MFMA", "1").startswith("0")
+KERNELS_2D = {"nc2c": ("nc2c_reg2d_kernel" if _FP32_GRIDDING else "nc2c_mfma2d_kernel") + " + nc2c_reduce2d_kernel",
+              "c2nc": "c2nc_multi2d_kernel" if _FP32_GRIDDING else "c2nc_mfma2d_kernel"}
+KERNELS_3D = {"nc2c": ("nc2c_reg3d_kernel" if _FP32_GRIDDING else "nc2c_mfma3d_kernel") + " + nc2c_reduce3d_kernel",
+              "c2nc": "c2nc_multi3d_kernel" if _FP32_GRIDDING else "c2nc_mfma3d_kernel"}
 KERNELS_GENERIC = {"nc2c": "nc2c_tile_kernel + nc2c_reduce_kernel", "c2nc": "c2nc_gather_kernel"}
 FP32_PEAK_TFLOPS = 157.3
 
