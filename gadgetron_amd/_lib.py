@@ -40,6 +40,9 @@ EXPORTED_SYMBOLS = [
     "gt_memcpy_dtod",
     "gt_memset",
     "gt_array_add",
+    "gt_array_sqrt",
+    "gt_array_scale",
+    "gt_array_mul_weights",
     "gt_stream_synchronize",
     "gt_device_synchronize",
     # include/gt_cgsense.h
@@ -185,6 +188,9 @@ def lib():
         "gt_memcpy_dtod": [P, P, S, P],
         "gt_memset": [P, I, S, P],
         "gt_array_add": [P, P, S, I, P],
+        "gt_array_sqrt": [P, S, P],
+        "gt_array_scale": [P, S, ctypes.c_float, P],
+        "gt_array_mul_weights": [P, P, S, S, P],
         "gt_stream_synchronize": [P],
         "gt_device_synchronize": [],
         "gt_sense_create": [PP, P, P, S, P, S],

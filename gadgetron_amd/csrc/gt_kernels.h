@@ -748,6 +748,26 @@ __global__ __launch_bounds__(256) void add_kernel(T* __restrict__ y, const T* __
         y[i] += x[i];
 }
 
+// y = sqrt(y), y *= a, x[b][i] *= w[i]: grid-stride element-wise kernels of
+// gt_array_sqrt, gt_array_scale and gt_array_mul_weights.
+__global__ __launch_bounds__(256) void sqrt_kernel(float* __restrict__ y, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = sqrtf(y[i]);
+}
+__global__ __launch_bounds__(256) void scale_kernel(float* __restrict__ y, long long n, float a) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] *= a;
+}
+__global__ __launch_bounds__(256) void mul_weights_kernel(float2* __restrict__ x, const float* __restrict__ w,
+                                                          long long n, long long total) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float a = w[i % n];
+        float2 v = x[i];
+        x[i] = make_float2(v.x * a, v.y * a);
+    }
+}
+
 __global__ __launch_bounds__(256) void sign_scale_kernel(float2* __restrict__ data, long long nb,
                                                          GridGeom g, float scale, int sign) {
     const long long G = g.G;

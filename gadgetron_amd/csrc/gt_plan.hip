@@ -2623,6 +2623,37 @@ gt_status gt_array_add(void* y, const void* x, size_t n, int precision, gt_strea
         GT_HIP(hipGetLastError());
     });
 }
+gt_status gt_array_sqrt(float* y, size_t n, gt_stream_t s) {
+    return guarded([&] {
+        if (n == 0) return;
+        GT_REQUIRE(y, GT_ERR_INVALID_ARGUMENT, "null array");
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, size_t(1) << 20);
+        hipLaunchKernelGGL(gt::sqrt_kernel, dim3(blocks), dim3(256), 0, S(s), y, (long long)n);
+        GT_HIP(hipGetLastError());
+    });
+}
+gt_status gt_array_scale(float* y, size_t n, float a, gt_stream_t s) {
+    return guarded([&] {
+        if (n == 0) return;
+        GT_REQUIRE(y, GT_ERR_INVALID_ARGUMENT, "null array");
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, size_t(1) << 20);
+        hipLaunchKernelGGL(gt::scale_kernel, dim3(blocks), dim3(256), 0, S(s), y, (long long)n, a);
+        GT_HIP(hipGetLastError());
+    });
+}
+gt_status gt_array_mul_weights(void* x, const float* w, size_t n, size_t batches, gt_stream_t s) {
+    return guarded([&] {
+        if (n == 0 || batches == 0) return;
+        GT_REQUIRE(x && w, GT_ERR_INVALID_ARGUMENT, "null array");
+        GT_REQUIRE(n <= size_t(1) << 40 && batches <= (size_t(1) << 40) / n, GT_ERR_INVALID_ARGUMENT,
+                   "array too large");
+        const size_t total = n * batches;
+        const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, size_t(1) << 20);
+        hipLaunchKernelGGL(gt::mul_weights_kernel, dim3(blocks), dim3(256), 0, S(s), (float2*)x, w, (long long)n,
+                           (long long)total);
+        GT_HIP(hipGetLastError());
+    });
+}
 gt_status gt_stream_synchronize(gt_stream_t s) { return guarded([&] { GT_HIP(hipStreamSynchronize(S(s))); }); }
 gt_status gt_device_synchronize(void) { return guarded([&] { GT_HIP(hipDeviceSynchronize()); }); }
 

@@ -193,6 +193,19 @@ gt_status gt_memset(void* ptr, int value, size_t bytes, gt_stream_t stream);
  * (core/gpu/cuNDArray_operators.cu:85-88), which NFFTOperator's accumulating
  * applications use (NFFTOperator.hpp:22-33). */
 gt_status gt_array_add(void* y, const void* x, size_t n, int precision, gt_stream_t stream);
+/* y = sqrt(y) over n floats: sqrt_inplace (core/gpu/cuNDArray_elemwise.h:64),
+ * which gpuCgSenseGadget applies to the density weights of a job
+ * (gadgets/pmri/gpuCgSenseGadget.cpp:129). */
+gt_status gt_array_sqrt(float* y, size_t n, gt_stream_t stream);
+/* y *= a over n floats: cuNDArray operator*= with a scalar
+ * (core/gpu/cuNDArray_operators.h:112), as on the regulariser image
+ * (gpuCgSenseGadget.cpp:170). */
+gt_status gt_array_scale(float* y, size_t n, float a, gt_stream_t stream);
+/* x[b][i] *= w[i] for complex64 x [batches][n] and float w [n]: cuNDArray
+ * operator*= of a complex array by a real array of its leading dimensions,
+ * batched over the rest (core/gpu/cuNDArray_operators.h:114-123), as on a
+ * job's samples (gpuCgSenseGadget.cpp:179). */
+gt_status gt_array_mul_weights(void* x, const float* w, size_t n, size_t batches, gt_stream_t stream);
 gt_status gt_stream_synchronize(gt_stream_t stream);
 gt_status gt_device_synchronize(void);
 

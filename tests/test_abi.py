@@ -98,3 +98,18 @@ def test_array_add_validates_without_touching_memory():
     assert L.gt_array_add(None, None, 4, 7, None) == 1
     assert L.gt_last_error()
     assert L.gt_array_add(None, None, 4, 0, None) == 1  # null arrays with n > 0
+
+
+def test_elementwise_ops_validate_without_touching_memory():
+    """gt_array_sqrt / gt_array_scale / gt_array_mul_weights (the CG-SENSE
+    gadget's sqrt of the weights, scalar scale and weighting of the samples):
+    empty arrays are no-ops, null arrays with elements are refused."""
+    L = lib()
+    assert L.gt_array_sqrt(None, 0, None) == 0
+    assert L.gt_array_scale(None, 0, 2.0, None) == 0
+    assert L.gt_array_mul_weights(None, None, 0, 3, None) == 0
+    assert L.gt_array_mul_weights(None, None, 5, 0, None) == 0
+    assert L.gt_array_sqrt(None, 4, None) == 1
+    assert L.gt_last_error()
+    assert L.gt_array_scale(None, 4, 2.0, None) == 1
+    assert L.gt_array_mul_weights(None, None, 4, 2, None) == 1
