@@ -890,6 +890,16 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
     const int sx = (t % g.ntiles[0]) * kRT - fr, sy0 = (t / g.ntiles[0]) * kRT - fr;
     const uint32_t ne = it.end - it.begin;  // at most 4 x 32
 
+    // this lane's sample (MFMA column) and its weight rows, loaded before the
+    // staging so that the two latencies overlap
+    const int col = lane & 31, hk = lane >> 5;
+    const uint32_t e = (uint32_t)(wave * 32 + col);
+    const bool on = e < ne;
+    const uint32_t j = list[it.begin + (on ? e : 0u)];
+    const float4* wr = reinterpret_cast<const float4*>(wtab + (size_t)(it.begin + (on ? e : 0u)) * 32);
+    const float4 xa = wr[2 * hk], xb = wr[2 * hk + 1];
+    const float4 y0 = wr[4], y1 = wr[5], y2 = wr[6], y3 = wr[7];
+
     // staging: thread = (cell column ex, coil), each cell's (re, im) read as
     // one 8-byte word; columns past E write zeros, so every staged row holds
     // 16 defined cells
@@ -913,6 +923,9 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
             v[ey][u] = (live && c0 + cl < C) ? prow[(long long)(c0 + cl) * cstride] : make_float2(0.0f, 0.0f);
         }
     }
+    const long long FM = (long long)F * M;
+    const int dm = dcw_mode(dcw, dcw_len, FM);
+    const float dj = dm == 1 ? dcw[j] : 1.0f;  // in flight with the staging
 #pragma unroll
     for (int ey = 0; ey < E; ey++) {
         const int gy = sy0 + ey;  // parity of the wrapped row equals that of gy (even N_os)
@@ -948,12 +961,6 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
     __syncthreads();
 
     if ((uint32_t)(wave * 32) >= ne) return;  // wave-uniform; no barrier follows
-    const int col = lane & 31, hk = lane >> 5;
-    const uint32_t e = (uint32_t)(wave * 32 + col);
-    const bool on = e < ne;
-    const float4* wr = reinterpret_cast<const float4*>(wtab + (size_t)(it.begin + (on ? e : 0u)) * 32);
-    const float4 xa = wr[2 * hk], xb = wr[2 * hk + 1];
-    const float4 y0 = wr[4], y1 = wr[5], y2 = wr[6], y3 = wr[7];
     const float z = on ? 1.0f : 0.0f;
     const mf_float8 wx = mf_float8{xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w} * z;
     const float wy[16] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w,
@@ -986,11 +993,7 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
     // accumulator register r of block a: row 32 a + 8 (r / 4) + 4 hk + r % 4 =
     // 2 coil + re/im, column = this lane's sample: (r, r + 1) for even r are
     // one coil's (re, im)
-    const long long FM = (long long)F * M;
-    const uint32_t j = list[it.begin + e];
-    const int dm = dcw_mode(dcw, dcw_len, FM);
-    float d = oscale / R;
-    if (dm == 1) d *= dcw[j];
+    const float d = oscale / R * dj;
     float2* o2 = reinterpret_cast<float2*>(out);
 #pragma unroll
     for (int a = 0; a < MB; a++)
