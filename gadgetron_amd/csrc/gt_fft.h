@@ -1,4 +1,4 @@
-// gt_fft.h -- the 2D centred FFT of the hot path, fused with pad / crop and
+// gt_fft.h -- the 2D and 3D centred FFTs of the hot path, fused with pad / crop and
 // deapodization (cuNFFT_impl::fft + deapodize, cuNFFT.cu:57-92, and the pad /
 // crop passes of NFFT_plan::compute, NFFT.hpp:125-184).
 //
@@ -393,6 +393,172 @@ __global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid
     for (int y = yr; y < OY; y += TPS) {
         if (BAND_OUT && (y < ylo || y >= yhi)) continue;
         col[y * rs] = sbuf[y * W + xl];
+    }
+}
+
+// ---- 3D band transforms ---------------------------------------------------
+// The 3D grid is the caller layout [batch][z][y][x].  The image occupies the
+// band [off, off + N) of every dimension, so
+//   forward : image lines (b, iz, iy) --pad, deapodize, sign, scale, x FFT-->
+//             grid lines (b, z, y) of the y and z bands;
+//             y FFT of the planes of the z band (y outside its band is zero);
+//             z FFT of every (y, x) line (z outside its band is zero)
+//   backward: z FFT of every (y, x) line, writing the z band;
+//             y FFT of the planes of the z band, writing the y band;
+//             x FFT of the lines of both bands --crop, deapodize, sign, scale-->
+//             image lines (b, iz, iy)
+// Lines and planes outside the bands are neither read nor written.
+struct Fft3Geom {
+    int Nx, Ny, Nz;        // image size
+    int Ox, Oy, Oz;        // grid size
+    int offx, offy, offz;  // band offsets
+    int nb;                // batches
+    const float* fx;       // reciprocal deapodization filters (length Ox, Oy, Oz)
+    const float* fy;
+    const float* fz;
+    float scale;           // 1/sqrt(G) times the centring global sign
+};
+
+// Forwards: image line r = (b * Nz + iz) * Ny + iy -> grid line (b, iz + offz,
+// iy + offy), padded at offx, deapodized, centring-signed, scaled, transformed.
+template <int OX>
+__global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft3_rows_pad_kernel(
+    const float2* __restrict__ img, float2* __restrict__ grid, Fft3Geom fg, const float2* __restrict__ tw,
+    float s) {
+    constexpr int NS = fft_slots<OX, true>();
+    __shared__ float2 sbuf[kFftRows][NS];
+    __shared__ float2 stw[OX];
+    load_twiddles<OX>(stw, tw);
+    const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
+    const long long r = (long long)blockIdx.x * kFftRows + rl;
+    const bool live = r < (long long)fg.nb * fg.Nz * fg.Ny;
+    const long long rr = live ? r : 0;
+    const int iy = (int)(rr % fg.Ny), iz = (int)((rr / fg.Ny) % fg.Nz);
+    const long long b = rr / ((long long)fg.Ny * fg.Nz);
+    const int y = iy + fg.offy, z = iz + fg.offz;
+    const float syz = fg.scale * fg.fy[y] * fg.fz[z] * (((y + z) & 1) ? -1.0f : 1.0f);
+    const float2* in = img + rr * fg.Nx;
+    float2* sb = &sbuf[rl][0];
+    constexpr int kPer = (OX + kFftRowThreads - 1) / kFftRowThreads;
+    float2 v[kPer];
+    float fxs[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+        const int x = t + i * kFftRowThreads;
+        const int ix = x - fg.offx;
+        const bool ok = live && x < OX && ix >= 0 && ix < fg.Nx;
+        v[i] = ok ? in[ix] : make_float2(0.0f, 0.0f);
+        fxs[i] = x < OX ? fg.fx[x] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+        const int x = t + i * kFftRowThreads;
+        if (x < OX) {
+            const float f = syz * fxs[i] * ((x & 1) ? -1.0f : 1.0f);
+            sb[fft_slot<true>(x, 1)] = make_float2(v[i].x * f, v[i].y * f);
+        }
+    }
+    __syncthreads();
+    fft_passes<OX, 1, kFftRowThreads, true>(sb, 1, t, stw, s);
+    if (!live) return;
+    float2* out = grid + ((b * fg.Oz + z) * fg.Oy + y) * (long long)OX;
+    for (int x = t; x < OX; x += kFftRowThreads) out[x] = sb[fft_slot<true>(x, 1)];
+}
+
+// Backwards: grid line (b, iz + offz, iy + offy), transformed along x, cropped
+// at offx, deapodized, centring-signed, scaled -> image line r.
+template <int OX>
+__global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft3_rows_crop_kernel(
+    const float2* __restrict__ grid, float2* __restrict__ img, Fft3Geom fg, const float2* __restrict__ tw,
+    float s) {
+    constexpr int NS = fft_slots<OX, true>();
+    __shared__ float2 sbuf[kFftRows][NS];
+    __shared__ float2 stw[OX];
+    load_twiddles<OX>(stw, tw);
+    const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
+    const long long r = (long long)blockIdx.x * kFftRows + rl;
+    const bool live = r < (long long)fg.nb * fg.Nz * fg.Ny;
+    const long long rr = live ? r : 0;
+    const int iy = (int)(rr % fg.Ny), iz = (int)((rr / fg.Ny) % fg.Nz);
+    const long long b = rr / ((long long)fg.Ny * fg.Nz);
+    const int y = iy + fg.offy, z = iz + fg.offz;
+    const float2* in = grid + ((b * fg.Oz + z) * fg.Oy + y) * (long long)OX;
+    float2* sb = &sbuf[rl][0];
+    {
+        constexpr int kPer = (OX + kFftRowThreads - 1) / kFftRowThreads;
+        float2 v[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int x = t + i * kFftRowThreads;
+            v[i] = (live && x < OX) ? in[x] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int x = t + i * kFftRowThreads;
+            if (x < OX) sb[fft_slot<true>(x, 1)] = v[i];
+        }
+    }
+    __syncthreads();
+    fft_passes<OX, 1, kFftRowThreads, true>(sb, 1, t, stw, s);
+    if (!live) return;
+    const float syz = fg.scale * fg.fy[y] * fg.fz[z] * (((y + z) & 1) ? -1.0f : 1.0f);
+    float2* out = img + rr * fg.Nx;
+    for (int ix = t; ix < fg.Nx; ix += kFftRowThreads) {
+        const int x = ix + fg.offx;
+        float2 v = sb[fft_slot<true>(x, 1)];
+        const float f = syz * fg.fx[x] * ((x & 1) ? -1.0f : 1.0f);
+        v.x *= f;
+        v.y *= f;
+        out[ix] = v;
+    }
+}
+
+// Strided lines of a 3D grid: line batch bb (of nbat) starts at
+// (bb / nsub) * s_outer + (bb % nsub + sub_off) * s_inner and holds ncols
+// adjacent lines whose elements are rs apart; elements outside [lo, hi) are
+// zero on input (BAND_IN) or not written (BAND_OUT).
+struct FftLines {
+    int nbat, nsub, sub_off, ncols;
+    long long s_outer, s_inner, rs;
+    int lo, hi;
+};
+
+// Transforms of length OL along strided lines, fft_col_width<OL>() adjacent
+// lines per workgroup, as fft_cols_kernel (and with its XCD pairing).
+template <int OL, bool BAND_IN, bool BAND_OUT>
+__global__ __launch_bounds__(256) void fft_lines_kernel(float2* __restrict__ grid, FftLines fl,
+                                                        const float2* __restrict__ tw, float s) {
+    constexpr int W = fft_col_width<OL>();
+    constexpr int TPS = 256 / W;
+    __shared__ float2 sbuf[OL * W];
+    __shared__ float2 stw[OL];
+    const int nxb = (fl.ncols + W - 1) / W;
+    int bb, xb;
+    if (!fft_col_block<W>((int)blockIdx.x, fl.nbat, nxb, bb, xb)) return;  // block-uniform, before any barrier
+    load_twiddles<OL>(stw, tw);
+    const int xl = threadIdx.x % W, yr = threadIdx.x / W;
+    const int x = xb * W + xl;
+    const bool xin = x < fl.ncols;
+    float2* line = grid + (long long)(bb / fl.nsub) * fl.s_outer + (long long)(bb % fl.nsub + fl.sub_off) * fl.s_inner +
+                   (xin ? x : 0);
+    {
+        static_assert(OL % TPS == 0, "line length must be a multiple of the threads per line");
+        constexpr int kPer = OL / TPS;
+        float2 v[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const int k = yr + i * TPS;
+            v[i] = (xin && (!BAND_IN || (k >= fl.lo && k < fl.hi))) ? line[k * fl.rs] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; i++) sbuf[(yr + i * TPS) * W + xl] = v[i];
+    }
+    __syncthreads();
+    fft_passes<OL, 1, TPS, false>(&sbuf[xl], W, yr, stw, s);
+    if (!xin) return;
+    for (int k = yr; k < OL; k += TPS) {
+        if (BAND_OUT && (k < fl.lo || k >= fl.hi)) continue;
+        line[k * fl.rs] = sbuf[k * W + xl];
     }
 }
 

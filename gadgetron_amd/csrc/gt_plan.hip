@@ -156,6 +156,7 @@ struct gt_nufft_plan {
     int nc2c_len_req = 0;  // NC2C entries per work item chosen by a fused batch's tuner (0: the rule below)
     bool fast2d = false;  // the 2D hot path (tables, row-interleaved grids)
     bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
+    bool fft_fused3 = false; // D == 3 (float) with every grid length among the fused FFT lengths
     bool fast3d = false;     // the 3D hot path (register-tile NC2C, staged C2NC)
     bool c2nc3_mfma = false; // fast3d with the matrix-core C2NC (home tiles of (9 - KW)^3 cells)
     bool dbl = false;        // GT_DOUBLE: the generic path in double precision
@@ -165,7 +166,7 @@ struct gt_nufft_plan {
     std::vector<float> filt_host[4];
     DevBuf filt[4];
     DevBuf filt_d[4];   // GT_DOUBLE: the filters in double
-    DevBuf twiddle[2];  // exp(-2 pi i m / N_os[d]), m < N_os[d] (fused FFT)
+    DevBuf twiddle[3];  // exp(-2 pi i m / N_os[d]), m < N_os[d] (fused FFT)
 
     // preprocess state
     size_t M = 0, F = 0;
@@ -1655,6 +1656,44 @@ struct LaunchColsBandOut {
     }
 };
 
+template <int L>
+struct LaunchRows3Pad {
+    static void run(const float2* img, float2* grid, const gt::Fft3Geom& fg, const float2* tw, float dir,
+                    hipStream_t s) {
+        const size_t rows = size_t(fg.nb) * fg.Nz * fg.Ny;
+        hipLaunchKernelGGL(gt::fft3_rows_pad_kernel<L>, dim3(unsigned((rows + gt::kFftRows - 1) / gt::kFftRows)),
+                           dim3(gt::kFftRowThreads * gt::kFftRows), 0, s, img, grid, fg, tw, dir);
+    }
+};
+template <int L>
+struct LaunchRows3Crop {
+    static void run(const float2* grid, float2* img, const gt::Fft3Geom& fg, const float2* tw, float dir,
+                    hipStream_t s) {
+        const size_t rows = size_t(fg.nb) * fg.Nz * fg.Ny;
+        hipLaunchKernelGGL(gt::fft3_rows_crop_kernel<L>, dim3(unsigned((rows + gt::kFftRows - 1) / gt::kFftRows)),
+                           dim3(gt::kFftRowThreads * gt::kFftRows), 0, s, grid, img, fg, tw, dir);
+    }
+};
+size_t lines_blocks(const gt::FftLines& fl, int W) {
+    size_t blocks = size_t(fl.nbat) * size_t((fl.ncols + W - 1) / W);
+    if (W == 8) blocks = (blocks + 15) / 16 * 16;  // whole groups of the XCD-paired order
+    return blocks;
+}
+template <int L>
+struct LaunchLinesBandIn {
+    static void run(float2* grid, const gt::FftLines& fl, const float2* tw, float dir, hipStream_t s) {
+        hipLaunchKernelGGL((gt::fft_lines_kernel<L, true, false>), dim3(unsigned(lines_blocks(fl, gt::fft_col_width<L>()))),
+                           dim3(256), 0, s, grid, fl, tw, dir);
+    }
+};
+template <int L>
+struct LaunchLinesBandOut {
+    static void run(float2* grid, const gt::FftLines& fl, const float2* tw, float dir, hipStream_t s) {
+        hipLaunchKernelGGL((gt::fft_lines_kernel<L, false, true>), dim3(unsigned(lines_blocks(fl, gt::fft_col_width<L>()))),
+                           dim3(256), 0, s, grid, fl, tw, dir);
+    }
+};
+
 gt::FftGeom fft_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float scale) {
     const gt::RowGeom rg = row_geom(p, nb, pad_side, scale);
     gt::FftGeom fg{};
@@ -1667,6 +1706,76 @@ gt::FftGeom fft_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float sca
     fg.fy = rg.fy;
     fg.scale = rg.scale;
     return fg;
+}
+
+gt::Fft3Geom fft3_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float scale) {
+    const gt::PadCrop pc = padcrop(p, pad_side, false, scale, 1);
+    gt::Fft3Geom fg{};
+    fg.Nx = pc.N[0];
+    fg.Ny = pc.N[1];
+    fg.Nz = pc.N[2];
+    fg.Ox = int(p->Nos[0]);
+    fg.Oy = int(p->Nos[1]);
+    fg.Oz = int(p->Nos[2]);
+    fg.offx = pc.off[0];
+    fg.offy = pc.off[1];
+    fg.offz = pc.off[2];
+    fg.nb = int(nb);
+    fg.fx = pc.filt[0];
+    fg.fy = pc.filt[1];
+    fg.fz = pc.filt[2];
+    fg.scale = scale;
+    return fg;
+}
+
+// 3D plans with the band transforms of gt_fft.h around the gridding (caller
+// layout [batch][z][y][x], image of the matrix size): forwards x (with pad and
+// deapodization), y over the planes of the z band, z over every line;
+// backwards the same in reverse, with the crop in the x pass.
+void compute_fused3(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const float* dcw,
+                    size_t dcw_len, int mode, float scale, hipStream_t s) {
+    const size_t nb = sh.nb;
+    GT_REQUIRE(nb * p->G() < (size_t(1) << 40) && nb * p->Nimg() / p->N[0] < (size_t(1) << 31) &&
+                   nb * p->N[2] * size_t((p->Nos[0] + 7) / 8) < (size_t(1) << 31) &&
+                   nb * size_t((p->Nos[0] * p->Nos[1] + 7) / 8) < (size_t(1) << 31),
+               GT_ERR_UNSUPPORTED, "too many batches for one launch");
+    p->grid.reserve(nb * p->G() * sizeof(float2));
+    float2* g = p->grid.as<float2>();
+    const gt::Layout L = caller_layout(p);
+    const int Ox = int(p->Nos[0]), Oy = int(p->Nos[1]), Oz = int(p->Nos[2]);
+    const float2* tw[3] = {p->twiddle[0].as<float2>(), p->twiddle[1].as<float2>(), p->twiddle[2].as<float2>()};
+    const bool c2nc = mode == GT_FORWARDS_C2NC || mode == GT_BACKWARDS_C2NC;
+    const gt::Fft3Geom fg = fft3_geom(p, nb, c2nc, scale);
+    const long long plane = (long long)Ox * Oy;
+    // y lines: the planes of the z band; z lines: every (y, x)
+    const gt::FftLines ly{int(nb) * fg.Nz, fg.Nz, fg.offz, Ox, (long long)Oz * plane, plane, Ox, fg.offy,
+                          fg.offy + fg.Ny};
+    const gt::FftLines lz{int(nb), 1, 0, int(plane), (long long)Oz * plane, 0, plane, fg.offz, fg.offz + fg.Nz};
+    if (c2nc) {
+        const float dir = mode == GT_FORWARDS_C2NC ? -1.0f : 1.0f;
+        {
+            StageTimer timer(p, GT_STAGE_FFT, s);
+            dispatch_fft_len<LaunchRows3Pad>(Ox, (const float2*)in, g, fg, tw[0], dir, s);
+            GT_HIP(hipGetLastError());
+            dispatch_fft_len<LaunchLinesBandIn>(Oy, g, ly, tw[1], dir, s);
+            GT_HIP(hipGetLastError());
+            dispatch_fft_len<LaunchLinesBandIn>(Oz, g, lz, tw[2], dir, s);
+            GT_HIP(hipGetLastError());
+        }
+        conv_c2nc(p, g, L, out, sh.C, 1, dcw, dcw_len, false, s);
+    } else {
+        const float dir = mode == GT_FORWARDS_NC2C ? -1.0f : 1.0f;
+        conv_nc2c(p, in, g, L, sh.C, 1, dcw, dcw_len, false, s);
+        {
+            StageTimer timer(p, GT_STAGE_FFT, s);
+            dispatch_fft_len<LaunchLinesBandOut>(Oz, g, lz, tw[2], dir, s);
+            GT_HIP(hipGetLastError());
+            dispatch_fft_len<LaunchLinesBandOut>(Oy, g, ly, tw[1], dir, s);
+            GT_HIP(hipGetLastError());
+            dispatch_fft_len<LaunchRows3Crop>(Ox, (const float2*)g, (float2*)out, fg, tw[0], dir, s);
+            GT_HIP(hipGetLastError());
+        }
+    }
 }
 
 // 2D hot path with the fused band transforms: pad + deapodize + row FFT over
@@ -1790,6 +1899,10 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
     const float scale = gs / std::sqrt(float(G));  // do_scale=true (NFFT.h:114-115, NFFT.hpp:205-238)
     if (p->fast2d && !sh.oversampled) {
         compute_fast2d(p, in, out, sh, dcw, dcw_len, mode, scale, s);
+        return;
+    }
+    if (p->fft_fused3 && !sh.oversampled && fft2d_strategy() == 2) {
+        compute_fused3(p, in, out, sh, dcw, dcw_len, mode, scale, s);
         return;
     }
     const gt::Layout L = caller_layout(p);
@@ -2265,9 +2378,11 @@ gt_status gt_nufft_create(gt_nufft_plan** out, const gt_nufft_desc* d) {
         }
         p->flag.reserve(sizeof(int));
         setup_geometry(p.get());
-        if (p->D == 2 && !p->dbl && fused_fft_len(int(p->Nos[0])) && fused_fft_len(int(p->Nos[1]))) {
-            p->fft_fused = true;
-            for (int i = 0; i < 2; i++) {
+        p->fft_fused = p->D == 2 && !p->dbl && fused_fft_len(int(p->Nos[0])) && fused_fft_len(int(p->Nos[1]));
+        p->fft_fused3 = p->D == 3 && !p->dbl && fused_fft_len(int(p->Nos[0])) && fused_fft_len(int(p->Nos[1])) &&
+                        fused_fft_len(int(p->Nos[2]));
+        if (p->fft_fused || p->fft_fused3) {
+            for (int i = 0; i < p->D; i++) {
                 const size_t L = p->Nos[i];
                 std::vector<float> tw(2 * L);
                 for (size_t m = 0; m < L; m++) {

@@ -99,6 +99,34 @@ def test_3d_split_tiles(gpu):
     check_both_ways([32, 32, 32], traj, 8, tj.ramp_dcf(traj, power=2), seed=75, max_items=40)
 
 
+@pytest.mark.parametrize("N,C,F", [([64, 96, 64], 3, 2), ([128, 64, 96], 9, 1)])
+def test_3d_band_fft_all_modes(gpu, N, C, F):
+    # grid sizes 96 / 144 / 192 (and 192 / 96 / 144): the band-limited 3D
+    # transforms of the library (x pass with pad or crop, y over the z band,
+    # z over every line), with 8- and 16-line workgroups along y and z
+    say(f"3D {N[0]} x {N[1]} x {N[2]}, {C} coils, {F} frames: every compute mode")
+    traj = frames_traj(120, 64, F, seed=31)
+    M = traj.shape[1]
+    rng = np.random.default_rng(91)
+    plan = NFFTPlan(N, None, W=5.5, os_factor=1.5)
+    plan.preprocess(dev(traj))
+    op = po.Plan(N, 5.5, alpha=1.5, precision="float")
+    op.preprocess(traj)
+    smp = tj.complex_normal(rng, (C, F, M))
+    img = tj.complex_normal(rng, (C, F) + tuple(N[::-1]))
+    dcw = tj.ramp_dcf(traj, power=2)
+    cases = [
+        (CM.FORWARDS_C2NC, img, (C, F, M), po.FORWARDS_C2NC),
+        (CM.BACKWARDS_NC2C, smp, img.shape, po.BACKWARDS_NC2C),
+        (CM.FORWARDS_NC2C, smp, img.shape, po.FORWARDS_NC2C),
+        (CM.BACKWARDS_C2NC, img, (C, F, M), po.BACKWARDS_C2NC),
+    ]
+    for mode, x, oshape, omode in cases:
+        err = rel_l2(run(plan, x, oshape, mode, dcw), op.compute(x, oshape, omode, dcw=dcw))
+        say(f"  mode {mode.name}: rel err {err:.2e}")
+        assert err <= TOL, mode.name
+
+
 @pytest.mark.parametrize("W", [3.5, 4.5, 6.5])
 def test_3d_window_widths(gpu, W):
     # windows of 4 and 5 cells: matrix-core C2NC tiles of 5^3 and 4^3 cells
