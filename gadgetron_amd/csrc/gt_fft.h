@@ -397,8 +397,9 @@ __global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid
 }
 
 // ---- 3D band transforms ---------------------------------------------------
-// The 3D grid is the caller layout [batch][z][y][x].  The image occupies the
-// band [off, off + N) of every dimension, so
+// The 3D grid has x contiguous and strides sb, sy, sz (in cells) for batch, y
+// and z (the gridding kernels' Layout).  The image occupies the band
+// [off, off + N) of every dimension, so
 //   forward : image lines (b, iz, iy) --pad, deapodize, sign, scale, x FFT-->
 //             grid lines (b, z, y) of the y and z bands;
 //             y FFT of the planes of the z band (y outside its band is zero);
@@ -413,6 +414,7 @@ struct Fft3Geom {
     int Ox, Oy, Oz;        // grid size
     int offx, offy, offz;  // band offsets
     int nb;                // batches
+    long long sb, sy, sz;  // grid strides of batch, y, z
     const float* fx;       // reciprocal deapodization filters (length Ox, Oy, Oz)
     const float* fy;
     const float* fz;
@@ -461,7 +463,7 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft3_rows_pad_kernel
     __syncthreads();
     fft_passes<OX, 1, kFftRowThreads, true>(sb, 1, t, stw, s);
     if (!live) return;
-    float2* out = grid + ((b * fg.Oz + z) * fg.Oy + y) * (long long)OX;
+    float2* out = grid + b * fg.sb + y * fg.sy + z * fg.sz;
     for (int x = t; x < OX; x += kFftRowThreads) out[x] = sb[fft_slot<true>(x, 1)];
 }
 
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft3_rows_crop_kerne
     const int iy = (int)(rr % fg.Ny), iz = (int)((rr / fg.Ny) % fg.Nz);
     const long long b = rr / ((long long)fg.Ny * fg.Nz);
     const int y = iy + fg.offy, z = iz + fg.offz;
-    const float2* in = grid + ((b * fg.Oz + z) * fg.Oy + y) * (long long)OX;
+    const float2* in = grid + b * fg.sb + y * fg.sy + z * fg.sz;
     float2* sb = &sbuf[rl][0];
     {
         constexpr int kPer = (OX + kFftRowThreads - 1) / kFftRowThreads;
@@ -514,11 +516,11 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft3_rows_crop_kerne
 }
 
 // Strided lines of a 3D grid: line batch bb (of nbat) starts at
-// (bb / nsub) * s_outer + (bb % nsub + sub_off) * s_inner and holds ncols
-// adjacent lines whose elements are rs apart; elements outside [lo, hi) are
-// zero on input (BAND_IN) or not written (BAND_OUT).
+// (bb / nsub + off_o) * s_outer + (bb % nsub + off_i) * s_inner and holds
+// ncols adjacent lines whose elements are rs apart; elements outside [lo, hi)
+// are zero on input (BAND_IN) or not written (BAND_OUT).
 struct FftLines {
-    int nbat, nsub, sub_off, ncols;
+    int nbat, nsub, off_o, off_i, ncols;
     long long s_outer, s_inner, rs;
     int lo, hi;
 };
@@ -539,8 +541,8 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(float2* __restrict__ gri
     const int xl = threadIdx.x % W, yr = threadIdx.x / W;
     const int x = xb * W + xl;
     const bool xin = x < fl.ncols;
-    float2* line = grid + (long long)(bb / fl.nsub) * fl.s_outer + (long long)(bb % fl.nsub + fl.sub_off) * fl.s_inner +
-                   (xin ? x : 0);
+    float2* line = grid + (long long)(bb / fl.nsub + fl.off_o) * fl.s_outer +
+                   (long long)(bb % fl.nsub + fl.off_i) * fl.s_inner + (xin ? x : 0);
     {
         static_assert(OL % TPS == 0, "line length must be a multiple of the threads per line");
         constexpr int kPer = OL / TPS;

@@ -1708,9 +1708,12 @@ gt::FftGeom fft_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float sca
     return fg;
 }
 
-gt::Fft3Geom fft3_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float scale) {
+gt::Fft3Geom fft3_geom(const gt_nufft_plan* p, size_t nb, const gt::Layout& L, bool pad_side, float scale) {
     const gt::PadCrop pc = padcrop(p, pad_side, false, scale, 1);
     gt::Fft3Geom fg{};
+    fg.sb = L.sb;
+    fg.sy = L.sy;
+    fg.sz = L.sz;
     fg.Nx = pc.N[0];
     fg.Ny = pc.N[1];
     fg.Nz = pc.N[2];
@@ -1728,29 +1731,31 @@ gt::Fft3Geom fft3_geom(const gt_nufft_plan* p, size_t nb, bool pad_side, float s
     return fg;
 }
 
-// 3D plans with the band transforms of gt_fft.h around the gridding (caller
-// layout [batch][z][y][x], image of the matrix size): forwards x (with pad and
-// deapodization), y over the planes of the z band, z over every line;
-// backwards the same in reverse, with the crop in the x pass.
+// 3D plans with the band transforms of gt_fft.h around the gridding (image of
+// the matrix size): forwards x (with pad and deapodization), y over the planes
+// of the z band, z over every line; backwards the same in reverse, with the
+// crop in the x pass.  The grid between them is [z][y][batch][x], so the coil
+// rows that the gridding kernels stage together for one (y, z) lie within
+// batches * N_os[0] cells of each other instead of a whole grid apart.
 void compute_fused3(gt_nufft_plan* p, const void* in, void* out, const Shape& sh, const float* dcw,
                     size_t dcw_len, int mode, float scale, hipStream_t s) {
     const size_t nb = sh.nb;
     GT_REQUIRE(nb * p->G() < (size_t(1) << 40) && nb * p->Nimg() / p->N[0] < (size_t(1) << 31) &&
                    nb * p->N[2] * size_t((p->Nos[0] + 7) / 8) < (size_t(1) << 31) &&
-                   nb * size_t((p->Nos[0] * p->Nos[1] + 7) / 8) < (size_t(1) << 31),
+                   nb * p->Nos[0] * p->Nos[1] < (size_t(1) << 31),
                GT_ERR_UNSUPPORTED, "too many batches for one launch");
     p->grid.reserve(nb * p->G() * sizeof(float2));
     float2* g = p->grid.as<float2>();
-    const gt::Layout L = caller_layout(p);
-    const int Ox = int(p->Nos[0]), Oy = int(p->Nos[1]), Oz = int(p->Nos[2]);
+    const gt::Layout L = internal_layout(p, nb);  // [z][y][batch][x]
+    const int Ox = int(p->Nos[0]);
     const float2* tw[3] = {p->twiddle[0].as<float2>(), p->twiddle[1].as<float2>(), p->twiddle[2].as<float2>()};
     const bool c2nc = mode == GT_FORWARDS_C2NC || mode == GT_BACKWARDS_C2NC;
-    const gt::Fft3Geom fg = fft3_geom(p, nb, c2nc, scale);
-    const long long plane = (long long)Ox * Oy;
-    // y lines: the planes of the z band; z lines: every (y, x)
-    const gt::FftLines ly{int(nb) * fg.Nz, fg.Nz, fg.offz, Ox, (long long)Oz * plane, plane, Ox, fg.offy,
-                          fg.offy + fg.Ny};
-    const gt::FftLines lz{int(nb), 1, 0, int(plane), (long long)Oz * plane, 0, plane, fg.offz, fg.offz + fg.Nz};
+    const gt::Fft3Geom fg = fft3_geom(p, nb, L, c2nc, scale);
+    const int Oy = int(p->Nos[1]), Oz = int(p->Nos[2]);
+    // y lines: batch bb = iz * nb + b of the z band, Ox lines each; z lines:
+    // one batch of every (y, batch, x) line
+    const gt::FftLines ly{int(nb) * fg.Nz, int(nb), fg.offz, 0, Ox, L.sz, L.sb, L.sy, fg.offy, fg.offy + fg.Ny};
+    const gt::FftLines lz{1, 1, 0, 0, int(L.sz), 0, 0, L.sz, fg.offz, fg.offz + fg.Nz};
     if (c2nc) {
         const float dir = mode == GT_FORWARDS_C2NC ? -1.0f : 1.0f;
         {
