@@ -1906,7 +1906,10 @@ void compute_impl(gt_nufft_plan* p, const void* in, const size_t* in_dims, int i
         compute_fast2d(p, in, out, sh, dcw, dcw_len, mode, scale, s);
         return;
     }
-    if (p->fft_fused3 && !sh.oversampled && fft2d_strategy() == 2) {
+    // the band transforms keep the grid as [z][y][batch][x], which only the
+    // 3D hot path's gridding kernels address (the generic and atomic kernels
+    // take the caller layout)
+    if (p->fast3d && p->fft_fused3 && !sh.oversampled && fft2d_strategy() == 2) {
         compute_fused3(p, in, out, sh, dcw, dcw_len, mode, scale, s);
         return;
     }

@@ -127,6 +127,33 @@ def test_3d_band_fft_all_modes(gpu, N, C, F):
         assert err <= TOL, mode.name
 
 
+@pytest.mark.parametrize("conv,W", [("ATOMIC", 5.5), ("STANDARD", 8.0)])
+def test_3d_band_lengths_generic_kernels(gpu, conv, W):
+    # 96^3 grids (band-FFT lengths) with gridding outside the 3D hot path:
+    # atomic NC2C, and a window of 8 cells (the generic kernels)
+    say(f"3D 64^3, 3 coils, {conv}, W {W}: every compute mode")
+    traj = tj.kooshball_3d(64, 90, seed=32)
+    M = traj.shape[1]
+    rng = np.random.default_rng(92)
+    plan = NFFTPlan([64, 64, 64], None, W=W, os_factor=1.5, conv=getattr(ga.ConvolutionType, conv))
+    plan.preprocess(dev(traj))
+    op = po.Plan([64, 64, 64], W, alpha=1.5, precision="float")
+    op.preprocess(traj)
+    C = 3
+    smp = tj.complex_normal(rng, (C, 1, M))
+    img = tj.complex_normal(rng, (C, 1, 64, 64, 64))
+    cases = [
+        (CM.FORWARDS_C2NC, img, (C, 1, M), po.FORWARDS_C2NC),
+        (CM.BACKWARDS_NC2C, smp, img.shape, po.BACKWARDS_NC2C),
+        (CM.FORWARDS_NC2C, smp, img.shape, po.FORWARDS_NC2C),
+        (CM.BACKWARDS_C2NC, img, (C, 1, M), po.BACKWARDS_C2NC),
+    ]
+    for mode, x, oshape, omode in cases:
+        err = rel_l2(run(plan, x, oshape, mode), op.compute(x, oshape, omode))
+        say(f"  mode {mode.name}: rel err {err:.2e}")
+        assert err <= TOL, mode.name
+
+
 @pytest.mark.parametrize("W", [3.5, 4.5, 6.5])
 def test_3d_window_widths(gpu, W):
     # windows of 4 and 5 cells: matrix-core C2NC tiles of 5^3 and 4^3 cells
