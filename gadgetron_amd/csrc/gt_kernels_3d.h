@@ -849,10 +849,11 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
 // ---------------------------------------------------------------------------
 // c2nc_mfma3d_kernel: the 3D C2NC of 8 coils per workgroup on the matrix cores.
 //
-// Samples are binned by home tile of T x T x T cells, T = c2nc_mfma3_tile(KW)
-// (3 for windows of 5 or 6 cells), so that a tile and its kernel halo span 8
-// staged cells per dimension (from tile origin - floor(r)).  For each staged plane z the samples of a tile are a
-// product,
+// Samples are binned by home tile of T x Ty x T cells, T = c2nc_mfma3_tile(KW)
+// (3 for windows of 5 or 6 cells) and Ty = c2nc_mfma3_tile(KW, EY), so that a
+// tile and its kernel halo span 8 staged cells in x and z and EY in y (from
+// tile origin - floor(r)).  For each staged plane z the samples of a tile are
+// a product,
 //   P_z[q][n] = sum over staged (x, y) of grid[q][x, y, z] * wx_n[x] * wy_n[y],
 // q = (coil, re/im), n = sample, and sample n of row q is sum_z wz_n[z] P_z[q][n].
 // A work item is a run of tiles along z in one column (tx, ty): the workgroup
@@ -860,7 +861,7 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
 // slots, so the next tile loads only its T new planes.  A plane is staged by
 // one wave, scaled by its own power of two R_z (largest |value| below 2^12)
 // and split into fp16 hi / lo rows; each wave then takes 16 samples at a time
-// and forms P_z as two K = 32 steps of v_mfma_f32_16x16x32_f16 (lane group
+// and forms P_z as EY / 4 K = 32 steps of v_mfma_f32_16x16x32_f16 (lane group
 // l / 16 takes staged row y = 4 h + l / 16 of step h, 8 x values per lane)
 // with the three products hi*hi + hi*lo + lo*hi of nc2c_mfma2d_kernel.  The B
 // operand wx wy is split once per 16 samples and serves all 8 planes; the z
@@ -868,12 +869,25 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
 // table's power-of-two x and y scales), then dcw.  Product order is fixed:
 // the result is bitwise repeatable.
 // ---------------------------------------------------------------------------
-constexpr int k3ME = 8;                             // staged cells per dimension
-constexpr int k3MAS = k3ME * k3ME * k3ME + 8;       // halves per LDS row (1040 bytes: 4 dwords past 32 banks)
-constexpr int k3MW = 24;                            // floats per weight-table entry
+constexpr int k3ME = 8;                             // staged cells in x and z
 constexpr int k3MWaves = 4;                         // wavefronts per workgroup
 constexpr int k3MSeg = 256;                         // entries per work item
-__host__ __device__ constexpr int c2nc_mfma3_tile(int KW) { return k3ME - c2nc_extent(0, KW); }
+// Staged rows in y, EY = 8, 12 or 16: home tiles of T x (EY - E0) x T cells
+// with T = 8 - E0, E0 = c2nc_extent(0, KW).  Taller tiles stage fewer halo
+// cells per sample (8 x EY cells for 3 x (EY - 5) tiles at KW 6) at EY / 8
+// times the matrix-core work per sample.  At 128^3 -> 192^3, 8 coils: EY 8
+// 1.59 ms, 12 1.35 ms, 16 1.63 ms (its 68 KB of LDS leave two workgroups
+// per CU).
+template <int EY>
+struct C2M3Geom {
+    static_assert(EY == 8 || EY == 12 || EY == 16, "staged rows in y");
+    static constexpr int Cells = k3ME * EY;      // cells of a staged plane
+    static constexpr int AS = k3ME * Cells + 8;  // halves per LDS row: 8 plane slots + 8
+    static constexpr int W = 2 * k3ME + EY;      // floats per weight-table entry: x, y, z
+    static constexpr int KS = EY / 4;            // K = 32 steps per plane
+    static constexpr int Occ = EY <= 12 ? 3 : 2; // workgroups per CU (LDS)
+};
+__host__ __device__ constexpr int c2nc_mfma3_tile(int KW, int E = k3ME) { return E - c2nc_extent(0, KW); }
 
 // Tile keys of the matrix-core 3D C2NC: tz fastest along a column (tx, ty),
 // the columns in bricks of 4 x 4 (x fastest within a brick, bricks x fastest),
@@ -908,12 +922,12 @@ __global__ void home_keys_column3_kernel(const float* __restrict__ p, long long 
     vals[j] = (uint32_t)j;
 }
 
-// Weight table of the matrix-core 3D C2NC, 24 floats per entry: the sample's x
-// weights at the 8 staged columns (scaled by sx), y weights at the 8 staged
-// rows (scaled by sy), z weights at the 8 staged planes; zero outside the
-// window.  sx and sy are powers of two that bring each axis's largest weight
-// to at most 1, so every x y product fits fp16.
-template <int KW>
+// Weight table of the matrix-core 3D C2NC, 16 + EY floats per entry: the
+// sample's x weights at the 8 staged columns (scaled by sx), y weights at the
+// EY staged rows (scaled by sy), z weights at the 8 staged planes; zero
+// outside the window.  sx and sy are powers of two that bring each axis's
+// largest weight to at most 1, so every x y product fits fp16.
+template <int KW, int EY>
 __global__ void c2nc_wtable3d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list, long long n,
                                      GridGeom g, float sx, float sy, float* __restrict__ wtab) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -921,7 +935,7 @@ __global__ void c2nc_wtable3d_kernel(const float* __restrict__ p, const uint32_t
     static_assert(KW >= 2, "a window of at least 2 cells");
     const uint32_t j = list[e];
     const int fr = (int)floorf(g.radius);
-    float r[3][k3ME];
+    float r[3][EY];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         float pw;
@@ -929,23 +943,24 @@ __global__ void c2nc_wtable3d_kernel(const float* __restrict__ p, const uint32_t
         float w[KW];
         int lo;
         window_raw<KW>(pw, g, d, w, lo);
-        // 0 <= b <= T; window slots past the 8 staged cells have weight 0
+        // 0 <= b <= T; window slots past the staged cells have weight 0
         const int b = min(max(lo - (td * g.tile[d] - fr), 0), g.tile[d]);
         const float sc = d == 0 ? sx : (d == 1 ? sy : 1.0f);
 #pragma unroll
-        for (int c = 0; c < k3ME; c++) r[d][c] = 0.0f;
+        for (int c = 0; c < EY; c++) r[d][c] = 0.0f;
 #pragma unroll
         for (int i = 0; i < KW; i++)
 #pragma unroll
-            for (int c = 0; c < k3ME; c++)
+            for (int c = 0; c < EY; c++)
                 if (c == b + i) r[d][c] = w[i] * sc;
     }
-    float4* t = reinterpret_cast<float4*>(wtab + e * k3MW);
+    float4* t = reinterpret_cast<float4*>(wtab + e * C2M3Geom<EY>::W);
+    t[0] = make_float4(r[0][0], r[0][1], r[0][2], r[0][3]);
+    t[1] = make_float4(r[0][4], r[0][5], r[0][6], r[0][7]);
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        t[2 * d] = make_float4(r[d][0], r[d][1], r[d][2], r[d][3]);
-        t[2 * d + 1] = make_float4(r[d][4], r[d][5], r[d][6], r[d][7]);
-    }
+    for (int q = 0; q < EY / 4; q++) t[2 + q] = make_float4(r[1][4 * q], r[1][4 * q + 1], r[1][4 * q + 2], r[1][4 * q + 3]);
+    t[2 + EY / 4] = make_float4(r[2][0], r[2][1], r[2][2], r[2][3]);
+    t[3 + EY / 4] = make_float4(r[2][4], r[2][5], r[2][6], r[2][7]);
 }
 
 // Largest of a non-negative value over the wavefront, in every lane: xor
@@ -988,11 +1003,13 @@ struct C2M3Column {
 };
 
 // One staged plane (unwrapped z coordinate zz) of a column in one wave: lane
-// = (x, coil) holds the plane's 8 rows y; a load instruction reads 64-byte
+// = (x, coil) holds the plane's EY rows y; a load instruction reads 64-byte
 // row segments.  The centring signs are kept as bits and applied when the
 // plane is split into LDS, so the loads stay in flight until then.
+template <int EY>
 struct C2M3Plane {
-    float2 v[k3ME];
+    using G = C2M3Geom<EY>;
+    float2 v[EY];
     uint32_t par;  // bit y: centring sign of row y
 
     __device__ __forceinline__ void load(const C2M3Column& c, int zz, const GridGeom& g, const Layout& L, int sign) {
@@ -1000,7 +1017,7 @@ struct C2M3Plane {
         const float2* pl = c.cb + L.sz * (long long)gz;
         par = 0;
 #pragma unroll
-        for (int y = 0; y < k3ME; y++) {
+        for (int y = 0; y < EY; y++) {
             int gy = c.sy + y;
             const int p = (c.px + gy + zz) & 1;
             gy = gy < 0 ? gy + g.nos[1] : (gy >= g.nos[1] ? gy - g.nos[1] : gy);
@@ -1015,7 +1032,7 @@ struct C2M3Plane {
     __device__ __forceinline__ void store(_Float16* Ahi, _Float16* Alo, float* inv, int slot, int lane) const {
         float m = 0.0f;
 #pragma unroll
-        for (int y = 0; y < k3ME; y++) m = fmaxf(m, fmaxf(fabsf(v[y].x), fabsf(v[y].y)));
+        for (int y = 0; y < EY; y++) m = fmaxf(m, fmaxf(fabsf(v[y].x), fabsf(v[y].y)));
         const float mx = wave_max_nonneg(m);
         int e2;
         frexpf(mx, &e2);               // mx < 2^e2, e2 <= 128 for finite values
@@ -1023,26 +1040,28 @@ struct C2M3Plane {
         const float R = mx > 0.0f ? ldexpf(1.0f, 12 - e2) : 1.0f;
         if (lane == 0) inv[slot] = 1.0f / R;  // a power of two: exact
         const int x = lane & 7, cl = lane >> 3;
-        _Float16* hr = Ahi + (2 * cl) * k3MAS + 64 * slot + x;
-        _Float16* lr = Alo + (2 * cl) * k3MAS + 64 * slot + x;
+        _Float16* hr = Ahi + (2 * cl) * G::AS + G::Cells * slot + x;
+        _Float16* lr = Alo + (2 * cl) * G::AS + G::Cells * slot + x;
 #pragma unroll
-        for (int y = 0; y < k3ME; y++) {
+        for (int y = 0; y < EY; y++) {
             const float sc = ((par >> y) & 1) ? -R : R;
             const float re = v[y].x * sc, im = v[y].y * sc;
             const _Float16 rh = (_Float16)re, ih = (_Float16)im;
             hr[8 * y] = rh;
             lr[8 * y] = (_Float16)(re - (float)rh);
-            hr[k3MAS + 8 * y] = ih;
-            lr[k3MAS + 8 * y] = (_Float16)(im - (float)ih);
+            hr[G::AS + 8 * y] = ih;
+            lr[G::AS + 8 * y] = (_Float16)(im - (float)ih);
         }
     }
 };
 
 // One block of 16 samples' weights (lane = sample l % 16, row group l / 16),
 // with the dcw of the sample's entry (dcw_e: per entry, or null).
+template <int EY>
 struct C2M3Weights {
+    using G = C2M3Geom<EY>;
     float4 x0, x1, z0, z1;
-    float wy0, wy1, d;
+    float wy[G::KS], d;
     uint32_t j;
     bool on;
 
@@ -1052,14 +1071,14 @@ struct C2M3Weights {
         const uint32_t e = bk * 16 + (uint32_t)n;
         on = e < ne;
         const uint32_t ee = e0 + (on ? e : 0u);  // ne > 0 here
-        const float* wr = wtab + (size_t)ee * k3MW;
+        const float* wr = wtab + (size_t)ee * G::W;
         const float4* w4 = reinterpret_cast<const float4*>(wr);
         x0 = w4[0];
         x1 = w4[1];
-        z0 = w4[4];
-        z1 = w4[5];
-        wy0 = wr[8 + kg];
-        wy1 = wr[12 + kg];
+        z0 = w4[2 + EY / 4];
+        z1 = w4[3 + EY / 4];
+#pragma unroll
+        for (int h = 0; h < G::KS; h++) wy[h] = wr[8 + 4 * h + kg];
         j = list[ee];
         d = dcw_e ? dcw_e[ee] : 1.0f;
     }
@@ -1086,16 +1105,17 @@ constexpr int k3MMaxTiles = k3MRec - 3;
 // staged.
 // Three waves per SIMD: registers for the plane and weights in flight (LDS
 // would hold four workgroups per CU).
-template <int MODE>
-__global__ __launch_bounds__(64 * k3MWaves, 3) void c2nc_mfma3d_kernel(
+template <int MODE, int EY>
+__global__ __launch_bounds__(64 * k3MWaves, C2M3Geom<EY>::Occ) void c2nc_mfma3d_kernel(
     const float* __restrict__ grid, float* __restrict__ out, const float* __restrict__ wtab,
     const uint32_t* __restrict__ list, const uint32_t* __restrict__ recs, int n_items, int M, int F, int C,
     GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
     const float* __restrict__ dcw_e, int accumulate, float oscale) {
-    __shared__ float4 smem[(2 * 16 * k3MAS * 2 + k3ME * 4 + 15) / 16];
+    using G = C2M3Geom<EY>;
+    __shared__ float4 smem[(2 * 16 * G::AS * 2 + k3ME * 4 + 15) / 16];
     _Float16* Ahi = reinterpret_cast<_Float16*>(smem);
-    _Float16* Alo = Ahi + 16 * k3MAS;
-    float* inv = reinterpret_cast<float*>(Alo + 16 * k3MAS);  // 1 / R of each plane slot
+    _Float16* Alo = Ahi + 16 * G::AS;
+    float* inv = reinterpret_cast<float*>(Alo + 16 * G::AS);  // 1 / R of each plane slot
     const int lane = threadIdx.x & 63, wave = wave_uniform(threadIdx.x >> 6);
     const int c0 = blockIdx.y * k3Coils;
     const int n = lane & 15, kg = lane >> 4;
@@ -1103,9 +1123,9 @@ __global__ __launch_bounds__(64 * k3MWaves, 3) void c2nc_mfma3d_kernel(
     // blocks of 16 samples go to the waves in reverse order (wave 3 first):
     // the new planes of a tile go to the first waves
     const uint32_t wb = (uint32_t)(k3MWaves - 1 - wave);
-    // A: row q = l % 16, staged cells (x = 0..7, y = 4 h + kg) of slot s at 64 s + 32 h + 8 kg
-    const _Float16* ah = Ahi + n * k3MAS + 8 * kg;
-    const _Float16* al = Alo + n * k3MAS + 8 * kg;
+    // A: row q = l % 16, staged cells (x = 0..7, y = 4 h + kg) of slot s at Cells s + 32 h + 8 kg
+    const _Float16* ah = Ahi + n * G::AS + 8 * kg;
+    const _Float16* al = Alo + n * G::AS + 8 * kg;
     const long long FM = (long long)F * M;
     float2* o2 = reinterpret_cast<float2*>(out);
     const float* de = MODE == 1 ? dcw_e : nullptr;
@@ -1115,7 +1135,7 @@ __global__ __launch_bounds__(64 * k3MWaves, 3) void c2nc_mfma3d_kernel(
     // planes of tile t of a column: t = 0 all 8 from sz0, then T new ones from sz0 + 8 + T (t - 1)
     auto first_plane = [&](const C2M3Column& c, int t) { return t == 0 ? c.sz0 : c.sz0 + k3ME + T * (t - 1); };
 
-    C2M3Plane pf;  // this wave's first new plane of the step ahead
+    C2M3Plane<EY> pf;  // this wave's first new plane of the step ahead
     // issues the loads of tile t of the column c
     auto prefetch = [&](const C2M3Column& c, int t) {
         const int np = t == 0 ? k3ME : T;
@@ -1139,12 +1159,12 @@ __global__ __launch_bounds__(64 * k3MWaves, 3) void c2nc_mfma3d_kernel(
         const int np = t == 0 ? k3ME : T;
         const int zb = first_plane(col, t);
         const uint32_t e0 = rec_at(rc, 2 + t), ne = rec_at(rc, 3 + t) - e0;
-        C2M3Weights w0;
+        C2M3Weights<EY> w0;
         if (wb * 16 < ne) w0.load(wtab, list, de, e0, ne, wb, n, kg);
         // split the new planes into their slots (every wave is past the previous gather)
         if (wave < np) pf.store(Ahi, Alo, inv, (zb + wave) & 7, lane);
         for (int j = wave + k3MWaves; j < np; j += k3MWaves) {
-            C2M3Plane q;
+            C2M3Plane<EY> q;
             q.load(col, zb + j, g, L, sign);
             q.store(Ahi, Alo, inv, (zb + j) & 7, lane);
         }
@@ -1173,14 +1193,14 @@ __global__ __launch_bounds__(64 * k3MWaves, 3) void c2nc_mfma3d_kernel(
         const int tz0 = col.sz0 + T * t;
         const uint32_t nb = (ne + 15) / 16;  // blocks of 16 samples
         for (uint32_t bk = wb; bk < nb; bk += k3MWaves) {
-            C2M3Weights w = w0;
+            C2M3Weights<EY> w = w0;
             if (bk != wb) w.load(wtab, list, de, e0, ne, bk, n, kg);  // the first was loaded before
             const mf_float8 wx =
                 mf_float8{w.x0.x, w.x0.y, w.x0.z, w.x0.w, w.x1.x, w.x1.y, w.x1.z, w.x1.w} * (w.on ? 1.0f : 0.0f);
             const float wz[k3ME] = {w.z0.x, w.z0.y, w.z0.z, w.z0.w, w.z1.x, w.z1.y, w.z1.z, w.z1.w};
-            mf_half8 bh0, bl0, bh1, bl1;
-            mf_split(wx * w.wy0, bh0, bl0);
-            mf_split(wx * w.wy1, bh1, bl1);
+            mf_half8 bh[G::KS], bl[G::KS];
+#pragma unroll
+            for (int h = 0; h < G::KS; h++) mf_split(wx * w.wy[h], bh[h], bl[h]);
             mf_float4 acc = mf_float4{0.0f, 0.0f, 0.0f, 0.0f};
             // an offset of 0 the compiler cannot see through, renewed per plane:
             // the A rows are read from LDS per block and plane rather than
@@ -1190,18 +1210,16 @@ __global__ __launch_bounds__(64 * k3MWaves, 3) void c2nc_mfma3d_kernel(
             for (int z = 0; z < k3ME; z++) {
                 asm volatile("" : "+v"(ao));
                 const int sl = (tz0 + z) & 7;
-                const int o = ao + 64 * sl;
-                const mf_half8 a0h = *reinterpret_cast<const mf_half8*>(ah + o);
-                const mf_half8 a0l = *reinterpret_cast<const mf_half8*>(al + o);
-                const mf_half8 a1h = *reinterpret_cast<const mf_half8*>(ah + o + 32);
-                const mf_half8 a1l = *reinterpret_cast<const mf_half8*>(al + o + 32);
+                const int o = ao + G::Cells * sl;
                 mf_float4 pz = mf_float4{0.0f, 0.0f, 0.0f, 0.0f};
-                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0l, bh0, pz, 0, 0, 0);
-                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, bl0, pz, 0, 0, 0);
-                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, bh0, pz, 0, 0, 0);
-                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, bh1, pz, 0, 0, 0);
-                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bl1, pz, 0, 0, 0);
-                pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bh1, pz, 0, 0, 0);
+#pragma unroll
+                for (int h = 0; h < G::KS; h++) {
+                    const mf_half8 a_h = *reinterpret_cast<const mf_half8*>(ah + o + 32 * h);
+                    const mf_half8 a_l = *reinterpret_cast<const mf_half8*>(al + o + 32 * h);
+                    pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_l, bh[h], pz, 0, 0, 0);
+                    pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_h, bl[h], pz, 0, 0, 0);
+                    pz = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_h, bh[h], pz, 0, 0, 0);
+                }
                 acc += (wz[z] * inv[sl]) * pz;  // column l % 16 = this lane's sample
             }
             if (!w.on) continue;

@@ -158,7 +158,8 @@ struct gt_nufft_plan {
     bool fft_fused = false;  // D == 2 with both grid lengths among the fused FFT lengths
     bool fft_fused3 = false; // D == 3 (float) with every grid length among the fused FFT lengths
     bool fast3d = false;     // the 3D hot path (register-tile NC2C, staged C2NC)
-    bool c2nc3_mfma = false; // fast3d with the matrix-core C2NC (home tiles of (9 - KW)^3 cells)
+    bool c2nc3_mfma = false; // fast3d with the matrix-core C2NC
+    int c2nc3_ey = 8;        // its staged rows in y (8, 12 or 16)
     bool dbl = false;        // GT_DOUBLE: the generic path in double precision
     double radius_d = 0, one_over_W_d = 0, beta_d[4] = {0, 0, 0, 0}, nos_d[4] = {1, 1, 1, 1};
 
@@ -775,11 +776,29 @@ struct LaunchTablesNC2C3 {
 template <int KW>
 struct LaunchWTableC2NC3 {
     static void run(const float* pos, const uint32_t* list, long long n, const gt::GridGeom& g, float sx, float sy,
-                    float* wtab, hipStream_t s) {
-        hipLaunchKernelGGL((gt::c2nc_wtable3d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos,
-                           list, n, g, sx, sy, wtab);
+                    float* wtab, int ey, hipStream_t s) {
+        const dim3 blocks((unsigned)((n + 255) / 256));
+        if (ey == 16)
+            hipLaunchKernelGGL((gt::c2nc_wtable3d_kernel<KW, 16>), blocks, dim3(256), 0, s, pos, list, n, g, sx, sy, wtab);
+        else if (ey == 12)
+            hipLaunchKernelGGL((gt::c2nc_wtable3d_kernel<KW, 12>), blocks, dim3(256), 0, s, pos, list, n, g, sx, sy, wtab);
+        else
+            hipLaunchKernelGGL((gt::c2nc_wtable3d_kernel<KW, 8>), blocks, dim3(256), 0, s, pos, list, n, g, sx, sy, wtab);
     }
 };
+
+// Staged rows in y of the matrix-core 3D C2NC: 12 (tiles of 3 x 7 x 3 cells
+// at windows of 5 or 6 cells); GT_NUFFT_C2NC3_EY = 8 or 16 selects the others.
+int c2nc3_rows() {
+    static const int ey = [] {
+        const char* e = std::getenv("GT_NUFFT_C2NC3_EY");
+        const int v = e ? std::atoi(e) : 12;
+        return (v == 8 || v == 16) ? v : 12;
+    }();
+    return ey;
+}
+
+int c2nc3_table_floats(int ey) { return 2 * gt::k3ME + ey; }
 
 template <int KW>
 struct LaunchTablesC2NC3 {
@@ -833,8 +852,14 @@ struct LaunchFast3C2NC {
             // registers), a multiple of the 8 XCDs so that every item stays on
             // the XCD its position assigns
             const float oscale = 1.0f / (kb_axis_scale(p, 0) * kb_axis_scale(p, 1));  // powers of two
-            auto km = mode == 0 ? gt::c2nc_mfma3d_kernel<0>
-                                : (mode == 1 ? gt::c2nc_mfma3d_kernel<1> : gt::c2nc_mfma3d_kernel<2>);
+            auto pick = [mode](auto k0, auto k1, auto k2) { return mode == 0 ? k0 : (mode == 1 ? k1 : k2); };
+            auto km = p->c2nc3_ey == 16
+                          ? pick(gt::c2nc_mfma3d_kernel<0, 16>, gt::c2nc_mfma3d_kernel<1, 16>,
+                                 gt::c2nc_mfma3d_kernel<2, 16>)
+                          : (p->c2nc3_ey == 12 ? pick(gt::c2nc_mfma3d_kernel<0, 12>, gt::c2nc_mfma3d_kernel<1, 12>,
+                                                      gt::c2nc_mfma3d_kernel<2, 12>)
+                                               : pick(gt::c2nc_mfma3d_kernel<0, 8>, gt::c2nc_mfma3d_kernel<1, 8>,
+                                                      gt::c2nc_mfma3d_kernel<2, 8>));
             int per_cu = 0;
             GT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, km, 64 * gt::k3MWaves, 0));
             const int resident = std::max(8, (std::max(per_cu, 1) * p->cus) & ~7);
@@ -961,7 +986,9 @@ void setup_geometry(gt_nufft_plan* p) {
             // the matrix-core C2NC stages tile + halo as 8 cells per dimension:
             // tiles of 3 cells for windows of 5 or 6 cells, 5 for 3 or 4
             const int tm = gt::c2nc_mfma3_tile(p->KW);
-            const int tmc[3] = {tm, tm, tm};
+            // taller tiles in y where the grid holds their staged rows within one wrap
+            p->c2nc3_ey = int(p->Nos[1]) >= c2nc3_rows() + p->KW ? c2nc3_rows() : 8;
+            const int tmc[3] = {tm, gt::c2nc_mfma3_tile(p->KW, p->c2nc3_ey), tm};
             p->c2nc3_mfma = gridding_on_matrix_cores() && tm >= 3;
             p->bc.geom = make_geom(p, p->c2nc3_mfma ? tmc : tc, false);
             if (p->c2nc3_mfma) {
@@ -1318,11 +1345,11 @@ void build_c2nc(gt_nufft_plan* p, hipStream_t s) {
         }
     } else if (p->c2nc3_mfma) {
         b.table.release();
-        b.wtab.reserve(std::max<size_t>(b.n, 1) * gt::k3MW * sizeof(float));
+        b.wtab.reserve(std::max<size_t>(b.n, 1) * size_t(c2nc3_table_floats(p->c2nc3_ey)) * sizeof(float));
         const float sx = kb_axis_scale(p, 0), sy = kb_axis_scale(p, 1);
         if (b.n)
             dispatch_fast<LaunchWTableC2NC3>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
-                                             b.geom, sx, sy, b.wtab.as<float>(), s);
+                                             b.geom, sx, sy, b.wtab.as<float>(), p->c2nc3_ey, s);
     } else {
         b.wtab.release();
         b.table.reserve(std::max<size_t>(b.n, 1) * gt::k3CDw * 4);
