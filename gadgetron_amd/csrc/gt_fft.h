@@ -525,12 +525,12 @@ struct FftLines {
     int lo, hi;
 };
 
-// Transforms of length OL along strided lines, fft_col_width<OL>() adjacent
-// lines per workgroup, as fft_cols_kernel (and with its XCD pairing).
-template <int OL, bool BAND_IN, bool BAND_OUT>
+// Transforms of length OL along strided lines, W adjacent lines per
+// workgroup (W = 8 with the XCD pairing of fft_cols_kernel; the 3D passes
+// take 16 or 32, whose runs are whole 128-byte lines already).
+template <int OL, bool BAND_IN, bool BAND_OUT, int W>
 __global__ __launch_bounds__(256) void fft_lines_kernel(float2* __restrict__ grid, FftLines fl,
                                                         const float2* __restrict__ tw, float s) {
-    constexpr int W = fft_col_width<OL>();
     constexpr int TPS = 256 / W;
     __shared__ float2 sbuf[OL * W];
     __shared__ float2 stw[OL];

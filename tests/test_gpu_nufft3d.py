@@ -99,11 +99,12 @@ def test_3d_split_tiles(gpu):
     check_both_ways([32, 32, 32], traj, 8, tj.ramp_dcf(traj, power=2), seed=75, max_items=40)
 
 
-@pytest.mark.parametrize("N,C,F", [([64, 96, 64], 3, 2), ([128, 64, 96], 9, 1)])
+@pytest.mark.parametrize("N,C,F", [([64, 96, 64], 3, 2), ([128, 64, 96], 9, 1), ([64, 192, 64], 2, 1)])
 def test_3d_band_fft_all_modes(gpu, N, C, F):
-    # grid sizes 96 / 144 / 192 (and 192 / 96 / 144): the band-limited 3D
-    # transforms of the library (x pass with pad or crop, y over the z band,
-    # z over every line), with 8- and 16-line workgroups along y and z
+    # grid sizes 96 / 144 / 192, 192 / 96 / 144 and 96 / 288 / 96: the
+    # band-limited 3D transforms of the library (x pass with pad or crop, y
+    # over the z band, z over every line), with 32-line workgroups along y and
+    # z up to length 240 and 16-line ones above
     say(f"3D {N[0]} x {N[1]} x {N[2]}, {C} coils, {F} frames: every compute mode")
     traj = frames_traj(120, 64, F, seed=31)
     M = traj.shape[1]
