@@ -169,8 +169,13 @@ __device__ __forceinline__ void fft_pass(float2* buf, int estride, int t, const 
             if (NS > 1) {
 #pragma unroll
                 for (int q = 1; q < R; q++) {
-                    float2 w = tw[k * q * (L / (NS * R))];  // k q < NS R: index below L
-                    w.y *= -s;  // the table holds e^{-i theta}; backwards conjugates
+                    // k q < NS R: index below L; the table holds its first half
+                    // (e^{-i theta} for m < L / 2; the second half is its negative)
+                    const int m = k * q * (L / (NS * R));
+                    const bool up = m >= L / 2;
+                    float2 w = tw[up ? m - L / 2 : m];
+                    if (up) w = make_float2(-w.x, -w.y);
+                    w.y *= -s;  // backwards conjugates
                     x[i][q] = cmul(x[i][q], w);
                 }
             }
@@ -193,10 +198,14 @@ __device__ __forceinline__ void fft_passes(float2* buf, int estride, int t, cons
     }
 }
 
-// Twiddle table of length L into LDS (all threads of the block).
+// First half of the twiddle table of length L (even) into LDS, all threads of
+// the block: e^{-2 pi i (m + L/2) / L} = -e^{-2 pi i m / L}, so the passes
+// need only m < L / 2 (half the LDS: the column transforms of length 384 fit
+// six workgroups per CU instead of five).
 template <int L>
 __device__ __forceinline__ void load_twiddles(float2* sw, const float2* __restrict__ tw) {
-    for (int i = threadIdx.x; i < L; i += blockDim.x) sw[i] = tw[i];
+    static_assert(L % 2 == 0, "even transform length");
+    for (int i = threadIdx.x; i < L / 2; i += blockDim.x) sw[i] = tw[i];
 }
 
 struct FftGeom {
@@ -247,7 +256,7 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_pad_kernel(
     float s) {
     constexpr int NS = fft_slots<OX, true>();
     __shared__ float2 sbuf[kFftRows][NS];
-    __shared__ float2 stw[OX];
+    __shared__ float2 stw[OX / 2];
     load_twiddles<OX>(stw, tw);
     const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
     const int r = blockIdx.x * kFftRows + rl;  // image row b * Ny + iy
@@ -314,7 +323,7 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft_rows_crop_kernel
     float s) {
     constexpr int NS = fft_slots<OX, true>();
     __shared__ float2 sbuf[kFftRows][NS];
-    __shared__ float2 stw[OX];
+    __shared__ float2 stw[OX / 2];
     load_twiddles<OX>(stw, tw);
     const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
     const int r = blockIdx.x * kFftRows + rl;
@@ -362,7 +371,7 @@ __global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid
     constexpr int W = fft_col_width<OY>();
     constexpr int TPS = 256 / W;
     __shared__ float2 sbuf[OY * W];
-    __shared__ float2 stw[OY];
+    __shared__ float2 stw[OY / 2];
     const int nxb = (Ox + W - 1) / W;
     int b, xb;
     if (!fft_col_block<W>((int)blockIdx.x, fg.nb, nxb, b, xb)) return;  // block-uniform, before any barrier
@@ -429,7 +438,7 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft3_rows_pad_kernel
     float s) {
     constexpr int NS = fft_slots<OX, true>();
     __shared__ float2 sbuf[kFftRows][NS];
-    __shared__ float2 stw[OX];
+    __shared__ float2 stw[OX / 2];
     load_twiddles<OX>(stw, tw);
     const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
     const long long r = (long long)blockIdx.x * kFftRows + rl;
@@ -475,7 +484,7 @@ __global__ __launch_bounds__(kFftRowThreads* kFftRows) void fft3_rows_crop_kerne
     float s) {
     constexpr int NS = fft_slots<OX, true>();
     __shared__ float2 sbuf[kFftRows][NS];
-    __shared__ float2 stw[OX];
+    __shared__ float2 stw[OX / 2];
     load_twiddles<OX>(stw, tw);
     const int rl = threadIdx.x / kFftRowThreads, t = threadIdx.x % kFftRowThreads;
     const long long r = (long long)blockIdx.x * kFftRows + rl;
@@ -533,7 +542,7 @@ __global__ __launch_bounds__(256) void fft_lines_kernel(float2* __restrict__ gri
                                                         const float2* __restrict__ tw, float s) {
     constexpr int TPS = 256 / W;
     __shared__ float2 sbuf[OL * W];
-    __shared__ float2 stw[OL];
+    __shared__ float2 stw[OL / 2];
     const int nxb = (fl.ncols + W - 1) / W;
     int bb, xb;
     if (!fft_col_block<W>((int)blockIdx.x, fl.nbat, nxb, bb, xb)) return;  // block-uniform, before any barrier
