@@ -181,10 +181,13 @@ __global__ void c2nc_wtable2d_kernel(const float* __restrict__ p, const uint32_t
 constexpr int kTabNW = 16;  // weight floats per NC2C entry (one 64-byte line)
 constexpr int kTabTile = 8;
 
+// The column weights are stored times csc, a power of two (the matrix-core
+// kernel's weight normaliser, so that it reads them ready to multiply; the
+// FP32 kernel scales its sums back by 1 / csc, exactly).
 template <int KW>
 __global__ void nc2c_table2d_kernel(const float* __restrict__ p, const uint32_t* __restrict__ list,
                                     const uint32_t* __restrict__ keys, long long n, GridGeom g,
-                                    float* __restrict__ wt, uint32_t* __restrict__ hd) {
+                                    float* __restrict__ wt, uint32_t* __restrict__ hd, float csc) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const uint32_t j = list[e];
@@ -202,7 +205,7 @@ __global__ void nc2c_table2d_kernel(const float* __restrict__ p, const uint32_t*
             if (w[i] == 0.0f) continue;
             const int l = wrap_cell(lo + i, g.nos[d]) - org[d];
             if (l >= 0 && l < kTabTile) {
-                tw[d][l] = w[i];
+                tw[d][l] = d == 0 ? w[i] * csc : w[i];
                 mask[d] |= 1u << l;
             }
         }

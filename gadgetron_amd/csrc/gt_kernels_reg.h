@@ -88,12 +88,14 @@ __device__ __forceinline__ void nc2c_chunk_load(const float* __restrict__ wt, co
 // follow the union of the groups' row masks; a row outside an entry's window
 // has weight 0); the groups' tiles are added in a fixed order at the end, so
 // the result stays bitwise repeatable.
+// wt's column weights carry the power of two csc (nc2c_table2d_kernel); the
+// sums are written times oscale = 1 / csc.
 template <int DM, int G>
 __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
     const float* __restrict__ samples, float* __restrict__ grid, float* __restrict__ partials,
     const float* __restrict__ wt, const uint32_t* __restrict__ hd, const WorkItem* __restrict__ items, int M,
     int F, int C, GridGeom g, Layout L, int sign, const float* __restrict__ dcw, long long dcw_len,
-    int accumulate) {
+    int accumulate, float oscale) {
     static_assert(G == 1 || G == 2 || G == 4, "coil groups");
     constexpr int CW = kCoilsPerWave / G;  // coils per wave
     constexpr int LC = CW / 4;             // coils per lane in a chunk's sample loads
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(64, 3) void nc2c_reg2d_kernel(
             const int cc = c0 + cl;
             const int gy = oy + ly;
             if (cc >= C || gx >= g.nos[0] || gy >= g.nos[1]) continue;
-            float v = xp[(((r & 3) * kRT) + lx) * kSubP + 2 * cl + comp];
+            float v = xp[(((r & 3) * kRT) + lx) * kSubP + 2 * cl + comp] * oscale;  // a power of two: exact
             if (it.partial >= 0) {
                 partials[(((long long)it.partial * C + cc) * (kRT * kRT) + ly * kRT + lx) * 2 + comp] = v;
                 continue;
@@ -735,7 +737,8 @@ __device__ __forceinline__ void nc2c_mfma2d_run(const MfGridArgs& a, int item, i
         const float4* p0 = reinterpret_cast<const float4*>(wb + (8 + cy) * WS);
         const float4* p1 = reinterpret_cast<const float4*>(wb + (12 + cy) * WS);
         const float4 c_0 = pc[0], c_1 = pc[1], r00 = p0[0], r01 = p0[1], r10 = p1[0], r11 = p1[1];
-        const mf_float8 wc = mf_float8{c_0.x, c_0.y, c_0.z, c_0.w, c_1.x, c_1.y, c_1.z, c_1.w} * wscale;
+        // the table's column weights carry wscale already
+        const mf_float8 wc = mf_float8{c_0.x, c_0.y, c_0.z, c_0.w, c_1.x, c_1.y, c_1.z, c_1.w};
         const mf_float8 w0 = wc * mf_float8{r00.x, r00.y, r00.z, r00.w, r01.x, r01.y, r01.z, r01.w};
         const mf_float8 w1 = wc * mf_float8{r10.x, r10.y, r10.z, r10.w, r11.x, r11.y, r11.z, r11.w};
         mf_half8 w0h, w0l, w1h, w1l;

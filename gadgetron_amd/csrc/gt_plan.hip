@@ -625,9 +625,9 @@ struct LaunchNC2CTileD {
 template <int KW>
 struct LaunchTablesNC2C {
     static void run(const float* pos, const uint32_t* list, const uint32_t* keys, long long n,
-                    const gt::GridGeom& g, float* wt, uint32_t* hd, hipStream_t s) {
+                    const gt::GridGeom& g, float* wt, uint32_t* hd, float csc, hipStream_t s) {
         hipLaunchKernelGGL((gt::nc2c_table2d_kernel<KW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           pos, list, keys, n, g, wt, hd);
+                           pos, list, keys, n, g, wt, hd, csc);
     }
 };
 
@@ -723,7 +723,8 @@ struct LaunchFastNC2C {
         hipLaunchKernelGGL(k, dim3((unsigned)p->bn.n_items, chunks), dim3(64), 0, s,
                            samples, grid, partials, p->bn.table.as<float>(), p->bn.heads.as<uint32_t>(),
                            p->bn.items.as<gt::WorkItem>(),
-                           int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw, dcw_len, accumulate);
+                           int(p->M), int(p->F), C, p->bn.geom, L, sign, dcw, dcw_len, accumulate,
+                           1.0f / mfma_weight_scale(p));
     }
 };
 
@@ -1299,7 +1300,7 @@ void build_nc2c(gt_nufft_plan* p, hipStream_t s) {
         if (b.n)
             dispatch_fast<LaunchTablesNC2C>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(),
                                             keys_sorted.as<uint32_t>(), (long long)b.n, b.geom,
-                                            b.table.as<float>(), b.heads.as<uint32_t>(), s);
+                                            b.table.as<float>(), b.heads.as<uint32_t>(), mfma_weight_scale(p), s);
         GT_HIP(hipGetLastError());
         make_items(p, b, counts, segment_length(p, counts, b.n, true), true, s);
         keys_sorted.release();
