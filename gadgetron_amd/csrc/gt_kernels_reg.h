@@ -845,8 +845,9 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_batch_kernel
 //
 // The samples of a home tile are a product: s[q][n] = sum over staged cells k
 // of grid[q][k] * w[k][n], q = (coil, re/im), n = sample, w[k][n] = wy_n[y] *
-// wx_n[x].  A workgroup of four waves stages the tile + halo (E x E cells, E =
-// c2nc_extent(8, KW), rows padded to 16 cells) once, scaled by a power of two R and
+// wx_n[x].  Home tiles are c2nc_mfma_tile_x(KW) x 8 cells, so that the tile +
+// halo is 16 columns (one K = 16 step per staged row) by E = c2nc_extent(8, KW)
+// rows.  A workgroup of four waves stages it once, scaled by a power of two R and
 // split into fp16 hi / lo planes in LDS; each wave then takes 32 samples and
 // walks the E staged rows as K = 16 steps of v_mfma_f32_32x32x16_f16 on
 // 2 CW / 32 blocks of 32 rows (16 coils each, rows 2 coil + re/im), with the three
@@ -856,6 +857,10 @@ __global__ __launch_bounds__(64, CW == 32 ? 2 : 3) void nc2c_mfma2d_batch_kernel
 // (oscale undoes the table's power-of-two weight scales), then dcw.
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int c2nc_mfma_row(int KW) { return c2nc_extent(kRT, KW) * 16 + 8; }  // halves per A row
+// Home tile width (x) of the matrix-core C2NC: its halo fills a staged row of
+// 16 cells (11 cells at windows of 5 or 6 cells, 9 at 7, 13 at 3 or 4); the
+// tiles are kRT rows high.
+__host__ __device__ constexpr int c2nc_mfma_tile_x(int KW) { return 16 - c2nc_extent(0, KW); }
 __host__ __device__ constexpr int c2nc_mfma_lds_bytes(int KW, int CW) {
     return 2 * (2 * CW) * c2nc_mfma_row(KW) * 2 + 64;
 }
@@ -875,7 +880,7 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
     const float* __restrict__ dcw = a.dcw;
     const long long dcw_len = a.dcw_len;
     const float oscale = a.scale;
-    constexpr int E = c2nc_extent(kRT, KW);  // staged cells per dimension
+    constexpr int E = c2nc_extent(kRT, KW);  // staged rows
     constexpr int AS = c2nc_mfma_row(KW);
     constexpr int MB = 2 * CW / 32;  // 32-row blocks
     static_assert(E <= 16 && (CW == 32 || CW == 16), "staged rows of 16 cells, 16 or 32 coils");
@@ -890,8 +895,9 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
     const int fr = (int)floorf(g.radius);
     const int f = (int)(it.key / (uint32_t)g.ntiles_total);
     const int t = (int)(it.key % (uint32_t)g.ntiles_total);
-    const int sx = (t % g.ntiles[0]) * kRT - fr, sy0 = (t / g.ntiles[0]) * kRT - fr;
+    const int sx = (t % g.ntiles[0]) * g.tile[0] - fr, sy0 = (t / g.ntiles[0]) * kRT - fr;
     const uint32_t ne = it.end - it.begin;  // at most 4 x 32
+    const int EX = c2nc_extent(g.tile[0], KW);  // staged columns (16 with c2nc_mfma_tile_x)
 
     // this lane's sample (MFMA column) and its weight rows, loaded before the
     // staging so that the two latencies overlap
@@ -904,11 +910,11 @@ __device__ __forceinline__ void c2nc_mfma2d_run(const MfGridArgs& a, int item, i
     const float4 y0 = wr[4], y1 = wr[5], y2 = wr[6], y3 = wr[7];
 
     // staging: thread = (cell column ex, coil), each cell's (re, im) read as
-    // one 8-byte word; columns past E write zeros, so every staged row holds
+    // one 8-byte word; columns past EX write zeros, so every staged row holds
     // 16 defined cells
     constexpr int kU = CW / 16;
     const int ex = tid & 15, cg = tid >> 4;
-    const bool live = ex < E;
+    const bool live = ex < EX;
     int gx = sx + ex;
     gx = gx < 0 ? gx + g.nos[0] : (gx >= g.nos[0] ? gx - g.nos[0] : gx);
     const long long cstride = L.sb * (long long)F;  // next coil, in cells

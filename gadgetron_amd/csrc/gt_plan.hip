@@ -183,6 +183,7 @@ struct gt_nufft_plan {
     DevBuf flag;
     Binning bn;   // NC2C: (tile, sample) pairs for every tile a window touches
     Binning bc;   // C2NC: one entry per sample in its home tile (2D hot path)
+    Binning bm;   // 2D matrix-core C2NC: home tiles whose halo fills 16-cell staged rows
 
     // workspaces
     DevBuf grid;
@@ -232,6 +233,7 @@ struct gt_nufft_plan {
         flag.release();
         bn.release();
         bc.release();
+        bm.release();
         grid.release();
         tmp_samples.release();
         partials.release();
@@ -241,7 +243,7 @@ struct gt_nufft_plan {
     void track() {
         for (DevBuf* b : {&traj, &traj_d, &row_inv, &sparse_dcw, &flag, &grid, &tmp_samples, &partials})
             b->epoch = &generation;
-        for (Binning* bb : {&bn, &bc})
+        for (Binning* bb : {&bn, &bc, &bm})
             for (DevBuf* b : {&bb->list, &bb->table, &bb->heads, &bb->dcw_e, &bb->wtab, &bb->items, &bb->red,
                               &bb->recs})
                 b->epoch = &generation;
@@ -741,7 +743,8 @@ struct LaunchFastC2NC {
         const int G = coil_groups(C);
         const int cw = gt::kCoilsPerWave / G;
         const unsigned chunks = (unsigned)((C + cw - 1) / cw);
-        if (G == 1 && b.wtab.ptr && gridding_on_matrix_cores()) {
+        if (G == 1 && p->bm.wtab.ptr && gridding_on_matrix_cores()) {
+            Binning& b = p->bm;
             // 17 coils or more: the home tiles' products on the matrix cores,
             // cwm coils per workgroup (9 to 16 coils measured faster on the
             // FP32 gather: 20.6 against 22.7 us at 256^2, 16 coils)
@@ -966,6 +969,13 @@ void setup_geometry(gt_nufft_plan* p) {
             p->fast2d = true;
             p->bn.geom = make_geom(p, ft);
             p->bc.geom = make_geom(p, ft);
+            // the matrix-core C2NC stages rows of 16 cells: home tiles of
+            // 16 - c2nc_extent(0, KW) columns (11 at windows of 5 or 6 cells)
+            // fill them; 8 rows as the FP32 kernel's.  Its staged columns run
+            // to tile origin - floor(r) + 15, within one wrap of the grid for
+            // N_os >= 32 (smaller grids keep 8-column tiles)
+            const int fm[3] = {int(p->Nos[0]) >= 32 ? gt::c2nc_mfma_tile_x(p->KW) : gt::kRT, gt::kRT, 1};
+            p->bm.geom = make_geom(p, fm);
         }
     }
     // 3D hot path: STANDARD gridding, a window of 3..7 cells, no requested
@@ -1334,15 +1344,21 @@ void build_c2nc(gt_nufft_plan* p, hipStream_t s) {
         b.table.reserve(std::max<size_t>(b.n, 1) * gt::kTabDw * 4);
         dispatch_fast<LaunchTablesC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
                                         b.geom, b.table.as<uint32_t>(), s);
+        b.wtab.release();
+        Binning& m = p->bm;
         if (gridding_on_matrix_cores()) {
-            // weight rows for the matrix-core C2NC (17 coils or more)
-            b.wtab.reserve(std::max<size_t>(b.n, 1) * 32 * sizeof(float));
+            // the matrix-core C2NC (17 coils or more) has its own binning, its
+            // home tiles wider, and per-sample weight rows
+            auto cm = bin_home(p, m.geom, m, s);
+            m.wtab.reserve(std::max<size_t>(m.n, 1) * 32 * sizeof(float));
             const float sx = kb_axis_scale(p, 0), sy = kb_axis_scale(p, 1);
-            if (b.n)
-                dispatch_fast<LaunchWTableC2NC>(p->KW, p->traj.as<float>(), b.list.as<uint32_t>(), (long long)b.n,
-                                                b.geom, sx, sy, b.wtab.as<float>(), s);
+            if (m.n)
+                dispatch_fast<LaunchWTableC2NC>(p->KW, p->traj.as<float>(), m.list.as<uint32_t>(), (long long)m.n,
+                                                m.geom, sx, sy, m.wtab.as<float>(), s);
+            GT_HIP(hipGetLastError());
+            make_items(p, m, cm, segment_length(p, cm, m.n, false), false, s);
         } else {
-            b.wtab.release();
+            m.release();
         }
     } else if (p->c2nc3_mfma) {
         b.table.release();
@@ -2038,7 +2054,7 @@ std::string gt_host::fused_unsupported(const FusedProblem& q) {
         return "the fused batch takes plans whose oversampled sizes are fused FFT lengths (96 ... 384)";
     if (p->conv_type != GT_CONV_STANDARD) return "the fused batch takes STANDARD convolution";
     if (!gridding_on_matrix_cores()) return "the fused batch runs on the matrix cores (GT_NUFFT_MFMA=0 is set)";
-    if (!(p->prep_c2nc && p->prep_nc2c) || !p->bc.wtab.ptr) return "plan not preprocessed for both directions";
+    if (!(p->prep_c2nc && p->prep_nc2c) || !p->bm.wtab.ptr) return "plan not preprocessed for both directions";
     if (q.C < 1 || !q.csm || !q.image || !q.data) return "missing coil maps, image or data";
     return "";
 }
@@ -2141,13 +2157,13 @@ gt_host::FusedGridding* gt_host::fused_create(const std::vector<FusedProblem>& p
                                        q.dcw, dl, 0, mfma_weight_scale(p)};
         ni[size_t(i)] = nd[size_t(i)];
         ni[size_t(i)].in = reinterpret_cast<const float*>(smp);
-        cn[size_t(i)] = gt::MfGridArgs{gridp, reinterpret_cast<float*>(smp), nullptr, p->bc.wtab.as<float>(),
-                                       p->bc.list.as<uint32_t>(), p->bc.items.as<gt::WorkItem>(), int(p->M),
-                                       int(p->F), C, p->bc.geom, L, 1, q.dcw, dl, 0,
+        cn[size_t(i)] = gt::MfGridArgs{gridp, reinterpret_cast<float*>(smp), nullptr, p->bm.wtab.as<float>(),
+                                       p->bm.list.as<uint32_t>(), p->bm.items.as<gt::WorkItem>(), int(p->M),
+                                       int(p->F), C, p->bm.geom, L, 1, q.dcw, dl, 0,
                                        1.0f / (kb_axis_scale(p, 0) * kb_axis_scale(p, 1))};
         const int chunks = (C + kFusedCW - 1) / kFusedCW;
         append_units(un, i, p->bn.n_items, chunks);
-        append_units(uc, i, p->bc.n_items, chunks);
+        append_units(uc, i, p->bm.n_items, chunks);
         reds[size_t(i)] = p->bn.red.as<gt::RedItem>();
         for (int k = 0; k < p->bn.n_red; k++) ur.push_back(gt::GridUnit{i, k, 0, 0});
         cmax = std::max(cmax, q.C);
