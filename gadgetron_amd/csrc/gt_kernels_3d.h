@@ -871,7 +871,7 @@ __global__ __launch_bounds__(64 * k3Waves, 4) void c2nc_multi3d_kernel(
 // ---------------------------------------------------------------------------
 constexpr int k3ME = 8;                             // staged cells in x and z
 constexpr int k3MWaves = 4;                         // wavefronts per workgroup
-constexpr int k3MSeg = 256;                         // entries per work item
+constexpr int k3MSeg = 2048;                        // entries per work item, at most
 // Staged rows in y, EY = 8, 12 or 16: home tiles of T x (EY - E0) x T cells
 // with T = 8 - E0, E0 = c2nc_extent(0, KW).  Taller tiles stage fewer halo
 // cells per sample (8 x EY cells for 3 x (EY - 5) tiles at KW 6) at EY / 8

@@ -1269,9 +1269,19 @@ void make_column_items(Binning& b, const std::vector<uint32_t>& counts, size_t L
 // partial tile write and read, splitting a C2NC tile one more halo staging.
 size_t segment_length(const gt_nufft_plan* p, const std::vector<uint32_t>& counts, size_t npairs,
                       bool nc2c) {
+    if (p->c2nc3_mfma && !nc2c) {
+        // the matrix-core 3D C2NC: an item restarts its column with 8 staged
+        // planes, so longer items stage less (128^3 kooshball, 8 coils: 256
+        // entries 1.37 ms, 512 1.25, 1024 1.21, 2048 1.19), but at least 3
+        // items per CU for small problems
+        const size_t target = npairs / size_t(3 * std::max(p->cus, 1));
+        size_t L = 256;
+        while (L * 2 <= std::min<size_t>(size_t(gt::k3MSeg), target)) L *= 2;
+        return p->max_items_req > 0 ? std::min(L, size_t(p->max_items_req)) : L;
+    }
     if ((p->fast2d || p->fast3d) && !nc2c) {
-        // the C2NC kernels stage a work item's whole table in LDS
-        const size_t cap = size_t(p->fast2d ? gt::kC2NCSeg : (p->c2nc3_mfma ? gt::k3MSeg : gt::k3Seg));
+        // the other C2NC kernels stage a work item's whole table in LDS
+        const size_t cap = size_t(p->fast2d ? gt::kC2NCSeg : gt::k3Seg);
         return p->max_items_req > 0 ? std::min(cap, size_t(p->max_items_req)) : cap;
     }
     if (p->nc2c_len_req > 0) return size_t(p->nc2c_len_req);
